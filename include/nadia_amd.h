@@ -1,0 +1,199 @@
+/*
+ * nadia_amd.h -- C ABI of the MI355X phase-retrieval library (libnadia_hip.so).
+ *
+ * This is the boundary behind NADIA's C++ reconstruction API: the drop-in
+ * classes (the headers in include/nadia: Complex_2D, Double_2D, BaseCDI, PlanarCDI,
+ * FresnelCDI, FresnelCDI_WF) call only these functions, and any other host
+ * language (ctypes, IDL, ...) can bind them directly.  Plain pointers and
+ * sizes only.
+ *
+ * Conventions
+ *   - Arrays cross the boundary as host pointers owned by the caller and are
+ *     copied.  Complex arrays are interleaved (re, im), real arrays plain;
+ *     both x-major: element (x, y) at x*ny + y, the in-memory layout of
+ *     NADIA's Complex_2D / Double_2D (NADIA/include/Complex_2D.h:179-194,
+ *     NADIA/include/Double_2D.h:121-143).  The element type is float for
+ *     NADIA_FP32 objects and double for NADIA_FP64 objects.
+ *   - Every function returns 0 on success and a negative code on error;
+ *     nadia_last_error() describes the last error of the calling thread.
+ *   - Device work is queued on one stream per device; calls that return host
+ *     data (downloads, get_error) wait for it.  An object must not be used by
+ *     two host threads at once (NADIA's own objects are not re-entrant either).
+ *   - Supported lengths per axis: 2^k (8..4096) and 3*2^k (24..3072).
+ */
+#ifndef NADIA_AMD_H
+#define NADIA_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision (NADIA's compile-time DOUBLE_PRECISION switch, include/types.h:5-39) */
+#define NADIA_FP32 0
+#define NADIA_FP64 1
+
+/* reconstruction kinds */
+#define NADIA_PLANAR 0     /* PlanarCDI      (NADIA/include/PlanarCDI.h:24-44) */
+#define NADIA_FRESNEL 1    /* FresnelCDI     (NADIA/include/FresnelCDI.h:33-232) */
+#define NADIA_FRESNEL_WF 2 /* FresnelCDI_WF  (NADIA/include/FresnelCDI_WF.h:31-186) */
+
+/* named algorithms, same values as NADIA/include/BaseCDI.h:43 */
+#define NADIA_ER 0
+#define NADIA_BIO 1
+#define NADIA_BOO 2
+#define NADIA_HIO 3
+#define NADIA_DM 4
+#define NADIA_SF 5
+#define NADIA_ASR 6
+#define NADIA_HPR 7
+#define NADIA_RAAR 8
+#define NADIA_CUSTOM 9
+
+/* execution paths of iterate() */
+#define NADIA_PATH_FUSED 0 /* two fused FFT passes per projection (default) */
+#define NADIA_PATH_PLAIN 1 /* one kernel per reference operation, reference data flow */
+
+/* error codes */
+#define NADIA_OK 0
+#define NADIA_EINVAL (-1)
+#define NADIA_EHIP (-2)
+#define NADIA_ESIZE (-3)
+#define NADIA_ESTATE (-4)
+
+typedef struct nadia_field nadia_field; /* device-resident complex field */
+typedef struct nadia_recon nadia_recon; /* one reconstruction (BaseCDI object) */
+
+const char* nadia_last_error(void);
+int nadia_version(void);
+int nadia_device_count(int* count);
+int nadia_set_device(int device);
+int nadia_size_supported(int n);
+
+/* ---- Complex_2D (NADIA/include/Complex_2D.h:35-515, src/Complex_2D.c++) ---- */
+
+/* ComplexR_2D(int,int) (src/Complex_2D.c++:16-32); zero-filled */
+int nadia_field_create(int nx, int ny, int precision, nadia_field** out);
+/* ~ComplexR_2D (src/Complex_2D.c++:88-98) */
+int nadia_field_destroy(nadia_field* f);
+int nadia_field_dims(const nadia_field* f, int* nx, int* ny, int* precision);
+/* host -> device / device -> host copies of the whole array */
+int nadia_field_upload(nadia_field* f, const void* host);
+int nadia_field_download(const nadia_field* f, void* host);
+/* copy (src/Complex_2D.c++:300-313) */
+int nadia_field_copy(nadia_field* dst, const nadia_field* src);
+/* perform_forward_fft (dir -1) / perform_backward_fft (+1), unnormalised, in
+ * place (src/Complex_2D.c++:486-503) */
+int nadia_field_fft(nadia_field* f, int direction);
+/* invert(bool scale) (src/Complex_2D.c++:317-353); even sizes */
+int nadia_field_invert(nadia_field* f, int scale);
+/* scale (src/Complex_2D.c++:169-180) */
+int nadia_field_scale(nadia_field* f, double factor);
+/* add(c2, scale) (src/Complex_2D.c++:183-199) */
+int nadia_field_add(nadia_field* f, const nadia_field* other, double scale);
+/* get_norm (src/Complex_2D.c++:259-270) */
+int nadia_field_norm(const nadia_field* f, double* out);
+/* get_2d(MAG, result) (src/Complex_2D.c++:158-165) into a host real array */
+int nadia_field_magnitude(const nadia_field* f, void* host_real);
+
+/* ---- BaseCDI (NADIA/include/BaseCDI.h:66-522, src/BaseCDI.c++) ---- */
+
+/* BaseCDI(Complex_2D&, n_best) (src/BaseCDI.c++:33-66) plus the subclass
+ * constructor for `kind`; `estimate` is referenced (not copied) and mutated
+ * by iterate(), like BaseCDI::complex. */
+int nadia_recon_create(nadia_field* estimate, int kind, int n_best, nadia_recon** out);
+int nadia_recon_destroy(nadia_recon* r);
+/* set_support(const Double_2D&, bool soften) (src/BaseCDI.c++:95-108) */
+int nadia_recon_set_support(nadia_recon* r, const void* host_real, int soften);
+/* set_intensity (src/BaseCDI.c++:118-124) */
+int nadia_recon_set_intensity(nadia_recon* r, const void* host_real);
+/* set_beam_stop (src/BaseCDI.c++:112-116) */
+int nadia_recon_set_beam_stop(nadia_recon* r, const void* host_real);
+/* set_algorithm (src/BaseCDI.c++:192-237), including its early return */
+int nadia_recon_set_algorithm(nadia_recon* r, int algorithm);
+/* set_custom_algorithm(m1..m10) (src/BaseCDI.c++:240-255) */
+int nadia_recon_set_custom_algorithm(nadia_recon* r, const double m[10]);
+/* set_relaxation_parameter (include/BaseCDI.h:264-267) */
+int nadia_recon_set_relaxation_parameter(nadia_recon* r, double beta);
+/* the five coefficients {PSF, PFS, PS, PF, PI} and the algorithm id
+ * (print_algorithm, src/BaseCDI.c++:278-295, prints these) */
+int nadia_recon_get_coefficients(const nadia_recon* r, double out5[5], int* algorithm);
+/* iterate() n times (src/BaseCDI.c++:297-387; FresnelCDI_WF::iterate,
+ * src/FresnelCDI_WF.c++:122-150).  errors (may be NULL) receives get_error()
+ * after each iteration; the estimate stays on the device. */
+int nadia_recon_iterate(nadia_recon* r, int n, double* errors);
+/* get_error (src/BaseCDI.c++:126-128) */
+int nadia_recon_get_error(nadia_recon* r, double* out);
+/* apply_shrinkwrap(gauss_width, threshold) (src/BaseCDI.c++:416-429) */
+int nadia_recon_apply_shrinkwrap(nadia_recon* r, double gauss_width, double threshold);
+/* get_support (include/BaseCDI.h:399-401) */
+int nadia_recon_get_support(const nadia_recon* r, void* host_real);
+/* initialise_estimate(seed): PlanarCDI (src/PlanarCDI.c++:39-62), FresnelCDI
+ * (src/FresnelCDI.c++:152-201), FresnelCDI_WF (src/FresnelCDI_WF.c++:99-117);
+ * uses the C library's srand/rand on the host, like the reference. */
+int nadia_recon_initialise_estimate(nadia_recon* r, int seed);
+/* public virtual operators on any field of the same size:
+ * apply_support (src/BaseCDI.c++:130-134), project_intensity (:156-160),
+ * scale_intensity (:163-189), propagate_to/from_detector (PlanarCDI.c++:65-73,
+ * FresnelCDI.c++:238-250, FresnelCDI_WF.c++:152-175) */
+int nadia_recon_apply_support(nadia_recon* r, nadia_field* c);
+int nadia_recon_project_intensity(nadia_recon* r, nadia_field* c);
+int nadia_recon_scale_intensity(nadia_recon* r, nadia_field* c);
+int nadia_recon_propagate_to_detector(nadia_recon* r, nadia_field* c);
+int nadia_recon_propagate_from_detector(nadia_recon* r, nadia_field* c);
+/* get_best_result(error, index) (src/BaseCDI.c++:82-91), reset_best
+ * (include/BaseCDI.h:476-479) */
+int nadia_recon_get_best_result(nadia_recon* r, int index, double* error, void* host_complex);
+int nadia_recon_reset_best(nadia_recon* r);
+/* PlanarCDI::get_intensity_autocorrelation (src/PlanarCDI.c++:13-36) */
+int nadia_recon_get_intensity_autocorrelation(nadia_recon* r, void* host_real);
+/* fused (default) or plain execution of iterate(); both compute the same
+ * reference algorithm */
+int nadia_recon_set_path(nadia_recon* r, int path);
+
+/* ---- FresnelCDI (NADIA/src/FresnelCDI.c++) ---- */
+
+/* FresnelCDI(initial_guess, white_field, wavelength, focal_detector_length,
+ * focal_sample_length, pixel_size, normalisation) (src/FresnelCDI.c++:20-44) */
+int nadia_fresnel_setup(nadia_recon* r, const void* white_field, double wavelength,
+                        double focal_detector_length, double focal_sample_length, double pixel_size,
+                        double normalisation);
+/* set_experimental_parameters (src/FresnelCDI.c++:57-95) */
+int nadia_fresnel_set_experimental_parameters(nadia_recon* r, double wavelength,
+                                              double focal_detector_length,
+                                              double focal_sample_length, double pixel_size);
+/* set_norm (src/FresnelCDI.c++:139-148), auto_set_norm (:129-137),
+ * set_normalisation (include/FresnelCDI.h:154-156, stores only) */
+int nadia_fresnel_set_norm(nadia_recon* r, double norm);
+int nadia_fresnel_auto_set_norm(nadia_recon* r);
+int nadia_fresnel_set_normalisation(nadia_recon* r, double norm);
+/* get_illumination_at_sample (src/FresnelCDI.c++:293-296) */
+int nadia_fresnel_get_illumination_at_sample(nadia_recon* r, void* host_complex);
+/* get_transmission_function (src/FresnelCDI.c++:298-355) of esw (NULL = the estimate) */
+int nadia_fresnel_get_transmission_function(nadia_recon* r, const nadia_field* esw, void* host_complex);
+/* set_transmission_function (src/FresnelCDI.c++:253-283) into esw (NULL = the estimate) */
+int nadia_fresnel_set_transmission_function(nadia_recon* r, const void* host_complex, nadia_field* esw);
+
+/* ---- FresnelCDI_WF (NADIA/src/FresnelCDI_WF.c++) ---- */
+
+/* FresnelCDI_WF(initial_guess, wavelength, zone_focal_length,
+ * focal_detector_length, pixel_size) (src/FresnelCDI_WF.c++:17-67) */
+int nadia_wf_setup(nadia_recon* r, double wavelength, double zone_focal_length,
+                   double focal_detector_length, double pixel_size);
+/* set_support(z_diameter, size) (src/FresnelCDI_WF.c++:177-196) */
+int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size);
+
+/* ---- measurement ---- */
+
+/* Record the device time of every fused pass launched by iterate() with HIP
+ * events on the library stream (on = 1); nadia_recon_pass_times returns, per
+ * pass kind (0 = column pass, 1 = row pass), the summed milliseconds and the
+ * number of launches since timing was switched on. */
+int nadia_recon_set_timing(nadia_recon* r, int on);
+int nadia_recon_pass_times(nadia_recon* r, double ms[2], long long count[2]);
+/* wait for all queued device work */
+int nadia_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NADIA_AMD_H */

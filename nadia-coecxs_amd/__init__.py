@@ -1,0 +1,412 @@
+"""Python mirror of NADIA's reconstruction API over the MI355X C ABI.
+
+The product is the C ABI in include/nadia_amd.h (libnadia_hip.so, HIP kernels
+for gfx950) and the C++ drop-in classes in include/nadia/.  This module is the
+Python view of the same objects, named as in the reference
+(NADIA/interfaces/python/pyNADIA/planarcdi.pyx:13-188 forwards the same calls):
+
+    est = Complex2D(nx, ny, double=True)
+    planar = PlanarCDI(est)                 # NADIA/include/PlanarCDI.h:24-44
+    planar.set_support(support); planar.set_intensity(intensity)
+    planar.initialise_estimate(0)
+    planar.set_algorithm(HIO)
+    planar.iterate(); planar.get_error(); planar.apply_shrinkwrap(1.5, 0.1)
+
+Arrays are numpy, x-major with shape (nx, ny) (element (x, y) at x*ny + y).
+There is no CPU fallback: every call runs on the GPU through the C ABI, and
+loading fails loudly when the HIP library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnadia_hip.so")
+
+FP32, FP64 = 0, 1
+PLANAR, FRESNEL, FRESNEL_WF = 0, 1, 2
+ER, BIO, BOO, HIO, DM, SF, ASR, HPR, RAAR, CUSTOM = range(10)
+PSF, PFS, PS, PF, PI = range(5)
+PATH_FUSED, PATH_PLAIN = 0, 1
+REAL, IMAG, MAG, PHASE, MAG_SQ = range(5)
+
+_ALG_NAMES = {"ER": ER, "BIO": BIO, "BOO": BOO, "HIO": HIO, "DM": DM, "SF": SF,
+              "ASR": ASR, "HPR": HPR, "RAAR": RAAR}
+
+_lib = None
+
+
+class NadiaError(RuntimeError):
+    pass
+
+
+def _sig(lib):
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    PP = ctypes.POINTER(ctypes.c_void_p)
+    IP = ctypes.POINTER(ctypes.c_int)
+    DP = ctypes.POINTER(ctypes.c_double)
+    LP = ctypes.POINTER(ctypes.c_longlong)
+    table = {
+        "nadia_last_error": (ctypes.c_char_p, []),
+        "nadia_version": (I, []),
+        "nadia_device_count": (I, [IP]),
+        "nadia_set_device": (I, [I]),
+        "nadia_size_supported": (I, [I]),
+        "nadia_synchronize": (I, []),
+        "nadia_field_create": (I, [I, I, I, PP]),
+        "nadia_field_destroy": (I, [P]),
+        "nadia_field_dims": (I, [P, IP, IP, IP]),
+        "nadia_field_upload": (I, [P, P]),
+        "nadia_field_download": (I, [P, P]),
+        "nadia_field_copy": (I, [P, P]),
+        "nadia_field_fft": (I, [P, I]),
+        "nadia_field_invert": (I, [P, I]),
+        "nadia_field_scale": (I, [P, D]),
+        "nadia_field_add": (I, [P, P, D]),
+        "nadia_field_norm": (I, [P, DP]),
+        "nadia_field_magnitude": (I, [P, P]),
+        "nadia_recon_create": (I, [P, I, I, PP]),
+        "nadia_recon_destroy": (I, [P]),
+        "nadia_recon_set_support": (I, [P, P, I]),
+        "nadia_recon_set_intensity": (I, [P, P]),
+        "nadia_recon_set_beam_stop": (I, [P, P]),
+        "nadia_recon_set_algorithm": (I, [P, I]),
+        "nadia_recon_set_custom_algorithm": (I, [P, DP]),
+        "nadia_recon_set_relaxation_parameter": (I, [P, D]),
+        "nadia_recon_get_coefficients": (I, [P, DP, IP]),
+        "nadia_recon_iterate": (I, [P, I, DP]),
+        "nadia_recon_get_error": (I, [P, DP]),
+        "nadia_recon_apply_shrinkwrap": (I, [P, D, D]),
+        "nadia_recon_get_support": (I, [P, P]),
+        "nadia_recon_initialise_estimate": (I, [P, I]),
+        "nadia_recon_apply_support": (I, [P, P]),
+        "nadia_recon_project_intensity": (I, [P, P]),
+        "nadia_recon_scale_intensity": (I, [P, P]),
+        "nadia_recon_propagate_to_detector": (I, [P, P]),
+        "nadia_recon_propagate_from_detector": (I, [P, P]),
+        "nadia_recon_get_best_result": (I, [P, I, DP, P]),
+        "nadia_recon_reset_best": (I, [P]),
+        "nadia_recon_get_intensity_autocorrelation": (I, [P, P]),
+        "nadia_recon_set_path": (I, [P, I]),
+        "nadia_fresnel_setup": (I, [P, P, D, D, D, D, D]),
+        "nadia_fresnel_set_experimental_parameters": (I, [P, D, D, D, D]),
+        "nadia_fresnel_set_norm": (I, [P, D]),
+        "nadia_fresnel_auto_set_norm": (I, [P]),
+        "nadia_fresnel_set_normalisation": (I, [P, D]),
+        "nadia_fresnel_get_illumination_at_sample": (I, [P, P]),
+        "nadia_fresnel_get_transmission_function": (I, [P, P, P]),
+        "nadia_fresnel_set_transmission_function": (I, [P, P, P]),
+        "nadia_wf_setup": (I, [P, D, D, D, D]),
+        "nadia_wf_set_support": (I, [P, D, D]),
+        "nadia_recon_set_timing": (I, [P, I]),
+        "nadia_recon_pass_times": (I, [P, DP, LP]),
+    }
+    for name, (res, args) in table.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+def library():
+    """Load libnadia_hip.so (built by __graft_entry__.build()); no fallback."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NadiaError("HIP library %s is not built (run __graft_entry__.build())" % LIB_PATH)
+        _lib = _sig(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise NadiaError(library().nadia_last_error().decode() or ("error %d" % rc))
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def synchronize():
+    _check(library().nadia_synchronize())
+
+
+def get_alg_from_name(name):
+    """BaseCDI::getAlgFromName (NADIA/include/BaseCDI.h:382-388): -1 if unknown."""
+    return _ALG_NAMES.get(name, -1)
+
+
+class Complex2D:
+    """Device-resident Complex_2D (NADIA/include/Complex_2D.h:35-515)."""
+
+    def __init__(self, nx, ny, double=True, data=None):
+        self.nx, self.ny = int(nx), int(ny)
+        self.double = bool(double)
+        self.cdt = np.complex128 if double else np.complex64
+        self.rdt = np.float64 if double else np.float32
+        h = ctypes.c_void_p()
+        _check(library().nadia_field_create(self.nx, self.ny, FP64 if double else FP32, ctypes.byref(h)))
+        self.h = h
+        if data is not None:
+            self.set(data)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.nadia_field_destroy(self.h)
+            self.h = None
+
+    def get_size_x(self):
+        return self.nx
+
+    def get_size_y(self):
+        return self.ny
+
+    def set(self, a):
+        a = np.ascontiguousarray(a, dtype=self.cdt)
+        assert a.shape == (self.nx, self.ny)
+        _check(library().nadia_field_upload(self.h, _ptr(a)))
+
+    def get(self):
+        out = np.empty((self.nx, self.ny), self.cdt)
+        _check(library().nadia_field_download(self.h, _ptr(out)))
+        return out
+
+    def copy(self, other):
+        _check(library().nadia_field_copy(self.h, other.h))
+
+    def perform_forward_fft(self):
+        _check(library().nadia_field_fft(self.h, -1))
+
+    def perform_backward_fft(self):
+        _check(library().nadia_field_fft(self.h, +1))
+
+    def invert(self, scale=False):
+        _check(library().nadia_field_invert(self.h, 1 if scale else 0))
+
+    def scale(self, factor):
+        _check(library().nadia_field_scale(self.h, float(factor)))
+
+    def add(self, other, scale=1.0):
+        _check(library().nadia_field_add(self.h, other.h, float(scale)))
+
+    def get_norm(self):
+        v = ctypes.c_double()
+        _check(library().nadia_field_norm(self.h, ctypes.byref(v)))
+        return v.value
+
+    def get_2d(self, kind=MAG):
+        if kind == MAG:
+            out = np.empty((self.nx, self.ny), self.rdt)
+            _check(library().nadia_field_magnitude(self.h, _ptr(out)))
+            return out
+        a = self.get()
+        if kind == REAL:
+            return a.real.copy()
+        if kind == IMAG:
+            return a.imag.copy()
+        if kind == PHASE:
+            return np.angle(a).astype(self.rdt)
+        if kind == MAG_SQ:
+            return (np.abs(a) ** 2).astype(self.rdt)
+        raise ValueError("unknown value type %r" % kind)
+
+
+class BaseCDI:
+    """BaseCDI (NADIA/include/BaseCDI.h:66-522) over nadia_recon."""
+
+    KIND = PLANAR
+
+    def __init__(self, estimate, n_best=0):
+        self.complex = estimate
+        self.nx, self.ny = estimate.nx, estimate.ny
+        self.rdt, self.cdt = estimate.rdt, estimate.cdt
+        h = ctypes.c_void_p()
+        _check(library().nadia_recon_create(estimate.h, self.KIND, int(n_best), ctypes.byref(h)))
+        self.h = h
+        self.n_best = n_best
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.nadia_recon_destroy(self.h)
+            self.h = None
+
+    def _real(self, a):
+        a = np.ascontiguousarray(a, dtype=self.rdt)
+        assert a.shape == (self.nx, self.ny), "array dimensions do not match"
+        return a
+
+    def get_size_x(self):
+        return self.nx
+
+    def get_size_y(self):
+        return self.ny
+
+    def set_support(self, support, soften=False):
+        _check(library().nadia_recon_set_support(self.h, _ptr(self._real(support)), int(bool(soften))))
+
+    def set_intensity(self, intensity):
+        _check(library().nadia_recon_set_intensity(self.h, _ptr(self._real(intensity))))
+
+    def set_beam_stop(self, beam_stop):
+        _check(library().nadia_recon_set_beam_stop(self.h, _ptr(self._real(beam_stop))))
+
+    def set_algorithm(self, alg):
+        _check(library().nadia_recon_set_algorithm(self.h, int(alg)))
+
+    def set_custom_algorithm(self, *m):
+        assert len(m) == 10
+        arr = (ctypes.c_double * 10)(*[float(v) for v in m])
+        _check(library().nadia_recon_set_custom_algorithm(self.h, arr))
+
+    def set_relaxation_parameter(self, beta):
+        _check(library().nadia_recon_set_relaxation_parameter(self.h, float(beta)))
+
+    def get_coefficients(self):
+        out = (ctypes.c_double * 5)()
+        alg = ctypes.c_int()
+        _check(library().nadia_recon_get_coefficients(self.h, out, ctypes.byref(alg)))
+        return np.array(list(out)), alg.value
+
+    def iterate(self, n=1):
+        """iterate() n times; returns the get_error() value of each iteration."""
+        errs = np.zeros(int(n))
+        _check(library().nadia_recon_iterate(self.h, int(n), errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return errs
+
+    def iterate_async(self, n=1):
+        """iterate() n times without copying the errors back (no host sync)."""
+        _check(library().nadia_recon_iterate(self.h, int(n), None))
+
+    def get_error(self):
+        v = ctypes.c_double()
+        _check(library().nadia_recon_get_error(self.h, ctypes.byref(v)))
+        return v.value
+
+    def apply_shrinkwrap(self, gauss_width=1.5, threshold=0.1):
+        _check(library().nadia_recon_apply_shrinkwrap(self.h, float(gauss_width), float(threshold)))
+
+    def get_support(self):
+        out = np.empty((self.nx, self.ny), self.rdt)
+        _check(library().nadia_recon_get_support(self.h, _ptr(out)))
+        return out
+
+    def initialise_estimate(self, seed=0):
+        _check(library().nadia_recon_initialise_estimate(self.h, int(seed)))
+
+    def apply_support(self, c):
+        _check(library().nadia_recon_apply_support(self.h, c.h))
+
+    def project_intensity(self, c):
+        _check(library().nadia_recon_project_intensity(self.h, c.h))
+
+    def scale_intensity(self, c):
+        _check(library().nadia_recon_scale_intensity(self.h, c.h))
+
+    def propagate_to_detector(self, c):
+        _check(library().nadia_recon_propagate_to_detector(self.h, c.h))
+
+    def propagate_from_detector(self, c):
+        _check(library().nadia_recon_propagate_from_detector(self.h, c.h))
+
+    def get_best_result(self, index=0):
+        out = np.empty((self.nx, self.ny), self.cdt)
+        err = ctypes.c_double()
+        rc = library().nadia_recon_get_best_result(self.h, int(index), ctypes.byref(err), _ptr(out))
+        if rc != 0:
+            return None, None
+        return out, err.value
+
+    def reset_best(self):
+        _check(library().nadia_recon_reset_best(self.h))
+
+    def set_path(self, path):
+        _check(library().nadia_recon_set_path(self.h, int(path)))
+
+    def set_timing(self, on=True):
+        _check(library().nadia_recon_set_timing(self.h, int(bool(on))))
+
+    def pass_times(self):
+        ms = (ctypes.c_double * 2)()
+        cnt = (ctypes.c_longlong * 2)()
+        _check(library().nadia_recon_pass_times(self.h, ms, cnt))
+        return list(ms), list(cnt)
+
+    @staticmethod
+    def getAlgFromName(name):
+        return get_alg_from_name(name)
+
+
+class PlanarCDI(BaseCDI):
+    """PlanarCDI (NADIA/include/PlanarCDI.h:24-44)."""
+
+    KIND = PLANAR
+
+    def get_intensity_autocorrelation(self):
+        out = np.empty((self.nx, self.ny), self.rdt)
+        _check(library().nadia_recon_get_intensity_autocorrelation(self.h, _ptr(out)))
+        return out
+
+
+class FresnelCDI(BaseCDI):
+    """FresnelCDI (NADIA/include/FresnelCDI.h:33-232)."""
+
+    KIND = FRESNEL
+
+    def __init__(self, estimate, white_field, beam_wavelength, focal_detector_length,
+                 focal_sample_length, pixel_size, normalisation=1.0, n_best=0):
+        super().__init__(estimate, n_best)
+        wf = np.ascontiguousarray(white_field, dtype=self.cdt)
+        assert wf.shape == (self.nx, self.ny)
+        _check(library().nadia_fresnel_setup(self.h, _ptr(wf), float(beam_wavelength),
+                                             float(focal_detector_length), float(focal_sample_length),
+                                             float(pixel_size), float(normalisation)))
+
+    def set_experimental_parameters(self, beam_wavelength, focal_detector_length, focal_sample_length,
+                                    pixel_size):
+        _check(library().nadia_fresnel_set_experimental_parameters(
+            self.h, float(beam_wavelength), float(focal_detector_length), float(focal_sample_length),
+            float(pixel_size)))
+
+    def set_norm(self, norm):
+        _check(library().nadia_fresnel_set_norm(self.h, float(norm)))
+
+    def auto_set_norm(self):
+        _check(library().nadia_fresnel_auto_set_norm(self.h))
+
+    def set_normalisation(self, norm):
+        _check(library().nadia_fresnel_set_normalisation(self.h, float(norm)))
+
+    def get_illumination_at_sample(self):
+        out = np.empty((self.nx, self.ny), self.cdt)
+        _check(library().nadia_fresnel_get_illumination_at_sample(self.h, _ptr(out)))
+        return out
+
+    def get_transmission_function(self, esw=None):
+        out = np.empty((self.nx, self.ny), self.cdt)
+        _check(library().nadia_fresnel_get_transmission_function(self.h, None if esw is None else esw.h,
+                                                                 _ptr(out)))
+        return out
+
+    def set_transmission_function(self, transmission, esw=None):
+        t = np.ascontiguousarray(transmission, dtype=self.cdt)
+        _check(library().nadia_fresnel_set_transmission_function(self.h, _ptr(t),
+                                                                 None if esw is None else esw.h))
+
+
+class FresnelCDI_WF(BaseCDI):
+    """FresnelCDI_WF (NADIA/include/FresnelCDI_WF.h:31-186)."""
+
+    KIND = FRESNEL_WF
+
+    def __init__(self, estimate, beam_wavelength, zone_focal_length, focal_detector_length, pixel_size,
+                 n_best=0):
+        super().__init__(estimate, n_best)
+        _check(library().nadia_wf_setup(self.h, float(beam_wavelength), float(zone_focal_length),
+                                        float(focal_detector_length), float(pixel_size)))
+
+    def set_support(self, z_diameter, size=1.01):  # hides BaseCDI::set_support like the reference
+        _check(library().nadia_wf_set_support(self.h, float(z_diameter), float(size)))

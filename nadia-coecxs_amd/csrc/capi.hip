@@ -1,0 +1,1462 @@
+// capi.hip -- implementation of the C ABI in include/nadia_amd.h.
+//
+// Device data of one reconstruction (nx x ny, real type T):
+//   X  : the caller's estimate (a nadia_field), complex, x-major
+//   W  : work field(s) of the fused path (2 fields when the algorithm has a
+//        P_F P_S term), complex
+//   S  : normalised support, real
+//   Q  : sqrt(I) with the beam-stop mask in the sign bit, real
+//   L  : (Fresnel) white field in the detector plane, Ls at the sample, complex
+//   t0..t3, conv : temporaries of the unfused path and of shrinkwrap
+// Fused iteration (Planar, Fresnel): column pass (forward column DFT, modulus
+// projection, inverse column DFT) then row pass (inverse row DFT, algorithm
+// update of X, forward row DFT of the new X); W holds the row DFT of X between
+// iterations.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/nadia_amd.h"
+#include "kernels.h"
+#include "plan.h"
+
+using namespace nadia;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess) return fail(NADIA_EHIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define LAUNCHCHK()                                                                         \
+  do {                                                                                      \
+    hipError_t e_ = hipGetLastError();                                                      \
+    if (e_ != hipSuccess) return fail(NADIA_EHIP, "kernel launch: %s", hipGetErrorString(e_)); \
+  } while (0)
+
+struct DeviceCtx {
+  hipStream_t stream = nullptr;
+  std::map<std::pair<int, int>, void*> twiddles;  // (n, precision) -> table
+};
+
+std::mutex g_mu;
+std::map<int, DeviceCtx> g_dev;
+
+int current_device() {
+  int d = 0;
+  (void)hipGetDevice(&d);
+  return d;
+}
+
+DeviceCtx* dev_ctx(int dev) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  DeviceCtx& c = g_dev[dev];
+  if (!c.stream) {
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  }
+  return &c;
+}
+
+// Twiddle table of the Stockham plan for length n: per stage s, L*(r-1)
+// values W_{L r}^{k m}, computed in long double and rounded once.
+void* twiddles(DeviceCtx* c, int n, int prec, bool row) {
+  const int p = row ? row_points(n) : col_points(n);
+  const int key_n = row ? n : -n;
+  auto key = std::make_pair(key_n, prec);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = c->twiddles.find(key);
+    if (it != c->twiddles.end()) return it->second;
+  }
+  PlanDesc d = make_plan(n, p);
+  if (d.nst <= 0) return nullptr;
+  const int total = plan_twiddles(d);
+  std::vector<double> re(total), im(total);
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int s = 0; s < d.nst; ++s) {
+    const int r = d.r[s], L = d.L[s];
+    for (int k = 0; k < L; ++k)
+      for (int m = 1; m < r; ++m) {
+        const long long t = ((long long)k * m) % ((long long)L * r);
+        const long double a = -twopi * (long double)t / (long double)(L * r);
+        re[d.tw_off[s] + k * (r - 1) + m - 1] = (double)cosl(a);
+        im[d.tw_off[s] + k * (r - 1) + m - 1] = (double)sinl(a);
+      }
+  }
+  void* dptr = nullptr;
+  size_t esz = prec == NADIA_FP64 ? 16 : 8;
+  if (hipMalloc(&dptr, (size_t)(total > 0 ? total : 1) * esz) != hipSuccess) return nullptr;
+  if (prec == NADIA_FP64) {
+    std::vector<double> h(2 * (size_t)total);
+    for (int i = 0; i < total; ++i) { h[2 * i] = re[i]; h[2 * i + 1] = im[i]; }
+    if (total) (void)hipMemcpy(dptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+  } else {
+    std::vector<float> h(2 * (size_t)total);
+    for (int i = 0; i < total; ++i) { h[2 * i] = (float)re[i]; h[2 * i + 1] = (float)im[i]; }
+    if (total) (void)hipMemcpy(dptr, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  c->twiddles[key] = dptr;
+  return dptr;
+}
+
+enum { A_PSF = 0, A_PFS = 1, A_PS = 2, A_PF = 3, A_PI = 4 };
+
+}  // namespace
+
+struct nadia_field {
+  int nx, ny, prec, dev;
+  size_t esz;  // bytes per complex element
+  void* d;
+  unsigned long long version;  // bumped by every write
+};
+
+struct nadia_recon {
+  nadia_field* est;
+  int kind, prec, nx, ny, dev, n_best;
+  DeviceCtx* ctx;
+  size_t esz, rsz;
+  long long npix;
+  void *S, *Q, *beam;
+  int has_beam, has_intensity;
+  void* W;  // 2 fields
+  void* t[4];
+  void* conv;
+  void* rtmp;
+  void* gwin;
+  double* partials;  // col-pass / reduction partial sums
+  double* errs;      // ring of per-iteration errors
+  double* scal;      // device scalars
+  int ring_cap;
+  double* last_err;  // device slot holding get_error()
+  double h_err_cache;
+  bool h_err_valid;
+  // algorithm state (BaseCDI)
+  double beta;
+  int algorithm;
+  double alg[5];
+  int path;
+  // fused-path caches
+  bool w_valid, w2_valid;
+  unsigned long long w_version, w2_version, w2_sversion, s_version;
+  // Fresnel
+  void *L, *Ls;
+  double norm, wl, zfd, zfs, pixel, phase_factor;
+  bool fresnel_ready;
+  // white field
+  double z12, z23, wf_sx, wf_sy, wf_k, wf_norm;
+  bool wf_ready;
+  // n best
+  std::vector<void*> best;
+  std::vector<double> best_err;
+  // timing
+  bool timing;
+  double pass_ms[2];
+  long long pass_n[2];
+  std::vector<hipEvent_t> ev_pool;
+};
+
+namespace {
+
+hipStream_t S_(const nadia_recon* r) { return r->ctx->stream; }
+
+template <typename T>
+int fft2d_T(DeviceCtx* c, void* buf, int nx, int ny, int dir) {
+  const int prec = sizeof(T) == 8 ? NADIA_FP64 : NADIA_FP32;
+  void* twy = twiddles(c, ny, prec, true);
+  void* twx = twiddles(c, nx, prec, false);
+  if (!twy || !twx) return fail(NADIA_EHIP, "twiddle table allocation failed");
+  RowArgs ra{};
+  ra.mode = ROW_FFT;
+  ra.nrows = nx;
+  ra.ny = ny;
+  ra.tw = twy;
+  ra.src = buf;
+  ra.dst = buf;
+  ra.dir = dir;
+  ra.scale = 1.0;
+  launch_row<T>(ra, c->stream);
+  LAUNCHCHK();
+  ColArgs ca{};
+  ca.mode = COL_FFT;
+  ca.nx = nx;
+  ca.ny = ny;
+  ca.tw = twx;
+  ca.buf = buf;
+  ca.dir = dir;
+  ca.scale = 1.0;
+  launch_col<T>(ca, 1, c->stream);
+  LAUNCHCHK();
+  return 0;
+}
+
+int fft2d(DeviceCtx* c, void* buf, int nx, int ny, int prec, int dir) {
+  return prec == NADIA_FP64 ? fft2d_T<double>(c, buf, nx, ny, dir) : fft2d_T<float>(c, buf, nx, ny, dir);
+}
+
+#define DISPATCH(prec, call) \
+  do {                       \
+    if ((prec) == NADIA_FP64) { using T = double; call; } \
+    else { using T = float; call; }                        \
+  } while (0)
+
+int check_dims(int nx, int ny) {
+  if (!size_supported(nx) || !size_supported(ny))
+    return fail(NADIA_ESIZE, "unsupported size %dx%d (supported per axis: 2^k 8..4096, 3*2^k 24..3072)", nx, ny);
+  return 0;
+}
+
+// ---- operators of one reconstruction on an arbitrary field buffer -------
+
+int op_invert(nadia_recon* r, void* buf) {
+  DISPATCH(r->prec, k_invert<T>(buf, r->nx, r->ny, 1, S_(r)));
+  LAUNCHCHK();
+  return 0;
+}
+
+int op_mulfac(nadia_recon* r, void* buf, int dir) {
+  if (r->kind == NADIA_FRESNEL)
+    DISPATCH(r->prec, k_mulfac_fresnel<T>(buf, r->nx, r->ny, r->phase_factor, dir, S_(r)));
+  else
+    DISPATCH(r->prec, k_mulfac_wf<T>(buf, r->nx, r->ny, r->wf_sx, r->wf_sy, r->z12, r->z23, r->wf_k, r->wf_norm,
+                                     dir, S_(r)));
+  LAUNCHCHK();
+  return 0;
+}
+
+int need_setup(nadia_recon* r) {
+  if (r->kind == NADIA_FRESNEL && !r->fresnel_ready)
+    return fail(NADIA_ESTATE, "FresnelCDI parameters not set (nadia_fresnel_setup)");
+  if (r->kind == NADIA_FRESNEL_WF && !r->wf_ready)
+    return fail(NADIA_ESTATE, "FresnelCDI_WF parameters not set (nadia_wf_setup)");
+  return 0;
+}
+
+int op_to_detector(nadia_recon* r, void* buf) {
+  int rc;
+  if ((rc = need_setup(r))) return rc;
+  if (r->kind == NADIA_PLANAR) {
+    if ((rc = fft2d(r->ctx, buf, r->nx, r->ny, r->prec, -1))) return rc;
+    return op_invert(r, buf);
+  }
+  if (r->kind == NADIA_FRESNEL) {
+    if ((rc = op_invert(r, buf))) return rc;
+    if ((rc = fft2d(r->ctx, buf, r->nx, r->ny, r->prec, -1))) return rc;
+    return op_mulfac(r, buf, -1);
+  }
+  if ((rc = fft2d(r->ctx, buf, r->nx, r->ny, r->prec, -1))) return rc;
+  if ((rc = op_mulfac(r, buf, -1))) return rc;
+  if ((rc = op_invert(r, buf))) return rc;
+  return fft2d(r->ctx, buf, r->nx, r->ny, r->prec, -1);
+}
+
+int op_from_detector(nadia_recon* r, void* buf) {
+  int rc;
+  if ((rc = need_setup(r))) return rc;
+  if (r->kind == NADIA_PLANAR) {
+    if ((rc = op_invert(r, buf))) return rc;
+    return fft2d(r->ctx, buf, r->nx, r->ny, r->prec, +1);
+  }
+  if (r->kind == NADIA_FRESNEL) {
+    if ((rc = op_mulfac(r, buf, +1))) return rc;
+    if ((rc = fft2d(r->ctx, buf, r->nx, r->ny, r->prec, +1))) return rc;
+    return op_invert(r, buf);
+  }
+  if ((rc = fft2d(r->ctx, buf, r->nx, r->ny, r->prec, +1))) return rc;
+  if ((rc = op_invert(r, buf))) return rc;
+  if ((rc = op_mulfac(r, buf, +1))) return rc;
+  return fft2d(r->ctx, buf, r->nx, r->ny, r->prec, +1);
+}
+
+// scale_intensity writing the error into *slot
+int op_scale_intensity(nadia_recon* r, void* buf, double* slot) {
+  if (!r->has_intensity) return fail(NADIA_ESTATE, "intensity not set");
+  int rc;
+  if ((rc = need_setup(r))) return rc;
+  if (r->kind == NADIA_FRESNEL) DISPATCH(r->prec, k_fresnel_add<T>(buf, r->L, 1.0, r->npix, S_(r)));
+  DISPATCH(r->prec, k_scale_intensity<T>(buf, r->Q, r->nx, r->ny, r->partials, kEwBlocks, S_(r)));
+  k_final_sum2(r->partials, kEwBlocks, slot, nullptr, S_(r));
+  if (r->kind == NADIA_FRESNEL) DISPATCH(r->prec, k_fresnel_add<T>(buf, r->L, -1.0, r->npix, S_(r)));
+  LAUNCHCHK();
+  r->last_err = slot;
+  r->h_err_valid = false;
+  return 0;
+}
+
+int op_project(nadia_recon* r, void* buf, double* slot) {
+  int rc;
+  if ((rc = op_to_detector(r, buf))) return rc;
+  if ((rc = op_scale_intensity(r, buf, slot))) return rc;
+  return op_from_detector(r, buf);
+}
+
+int op_support(nadia_recon* r, void* buf) {
+  DISPATCH(r->prec, k_support<T>(buf, r->S, r->npix, S_(r)));
+  LAUNCHCHK();
+  return 0;
+}
+
+int copy_field(nadia_recon* r, void* dst, const void* src) {
+  HIPCHK(hipMemcpyAsync(dst, src, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+  return 0;
+}
+
+// Unfused iteration following BaseCDI::iterate's data flow
+// (NADIA/src/BaseCDI.c++:297-387) one kernel per reference operation.
+int iterate_plain(nadia_recon* r, double* slot) {
+  int rc;
+  void* x = r->est->d;
+  if (r->kind == NADIA_FRESNEL_WF) {  // FresnelCDI_WF::iterate, src/FresnelCDI_WF.c++:122-150
+    if ((rc = op_from_detector(r, x))) return rc;
+    if ((rc = op_support(r, x))) return rc;
+    if ((rc = op_to_detector(r, x))) return rc;
+    return op_scale_intensity(r, x, slot);
+  }
+  if (r->algorithm == NADIA_ER) {
+    if ((rc = op_project(r, x, slot))) return rc;
+    return op_support(r, x);
+  }
+  void *tPSF = r->t[0], *tPFS = r->t[1], *tPS = r->t[2], *tPF = r->t[3];
+  const double* a = r->alg;
+  if (a[A_PFS] != 0) {
+    if ((rc = copy_field(r, tPFS, x))) return rc;
+    if ((rc = op_support(r, tPFS))) return rc;
+    if ((rc = op_project(r, tPFS, slot))) return rc;
+  }
+  if (a[A_PF] != 0) {
+    if ((rc = copy_field(r, tPF, x))) return rc;
+    if ((rc = op_project(r, tPF, slot))) return rc;
+  }
+  if (a[A_PS] != 0) {
+    if ((rc = copy_field(r, tPS, x))) return rc;
+    if ((rc = op_support(r, tPS))) return rc;
+  }
+  if (a[A_PSF] != 0) {
+    if (a[A_PF] != 0) {
+      if ((rc = copy_field(r, tPSF, tPF))) return rc;
+      if ((rc = op_support(r, tPSF))) return rc;
+    } else {
+      if ((rc = copy_field(r, tPSF, x))) return rc;
+      if ((rc = op_project(r, tPSF, slot))) return rc;
+      if ((rc = op_support(r, tPSF))) return rc;
+    }
+  }
+  // coefficient order expected by k_combine: a(PSF), b(PFS), c(PS), d(PF), e(PI)
+  double coef[5] = {a[A_PSF], a[A_PFS], a[A_PS], a[A_PF], a[A_PI]};
+  DISPATCH(r->prec, k_combine<T>(x, tPFS, tPF, tPS, tPSF, coef, r->npix, S_(r)));
+  LAUNCHCHK();
+  return 0;
+}
+
+bool has_projection(const nadia_recon* r) {
+  if (r->kind == NADIA_FRESNEL_WF || r->algorithm == NADIA_ER) return true;
+  const double* a = r->alg;
+  return a[A_PF] != 0 || a[A_PSF] != 0 || a[A_PFS] != 0;
+}
+
+bool fused_ok(const nadia_recon* r) { return r->path == NADIA_PATH_FUSED && has_projection(r); }
+
+struct Ev {
+  hipEvent_t a, b;
+  int kind;
+};
+
+int record_start(nadia_recon* r, std::vector<Ev>& evs, int kind) {
+  if (!r->timing) return 0;
+  Ev e{};
+  HIPCHK(hipEventCreate(&e.a));
+  HIPCHK(hipEventCreate(&e.b));
+  e.kind = kind;
+  HIPCHK(hipEventRecord(e.a, S_(r)));
+  evs.push_back(e);
+  return 0;
+}
+int record_stop(nadia_recon* r, std::vector<Ev>& evs) {
+  if (!r->timing) return 0;
+  HIPCHK(hipEventRecord(evs.back().b, S_(r)));
+  return 0;
+}
+int collect_events(nadia_recon* r, std::vector<Ev>& evs) {
+  if (evs.empty()) return 0;
+  HIPCHK(hipEventSynchronize(evs.back().b));
+  for (auto& e : evs) {
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+    r->pass_ms[e.kind] += ms;
+    r->pass_n[e.kind] += 1;
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  evs.clear();
+  return 0;
+}
+
+template <typename T>
+int fused_prepare(nadia_recon* r, bool two) {
+  void* twy = twiddles(r->ctx, r->ny, r->prec, true);
+  if (!twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
+  const long long n = r->npix;
+  if (!r->w_valid || r->w_version != r->est->version) {
+    RowArgs ra{};
+    ra.mode = ROW_FFT;
+    ra.nrows = r->nx;
+    ra.ny = r->ny;
+    ra.tw = twy;
+    ra.src = r->est->d;
+    ra.dst = r->W;
+    ra.scale = 1.0;
+    if (r->kind == NADIA_FRESNEL_WF) {
+      ra.dir = +1;
+      ra.pre_sign = 1;
+    } else {
+      ra.dir = -1;
+    }
+    launch_row<T>(ra, S_(r));
+    LAUNCHCHK();
+    r->w_valid = true;
+    r->w_version = r->est->version;
+  }
+  if (two && (!r->w2_valid || r->w2_version != r->est->version || r->w2_sversion != r->s_version)) {
+    void* W2 = (char*)r->W + n * r->esz;
+    HIPCHK(hipMemcpyAsync(W2, r->est->d, n * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+    k_support<T>(W2, r->S, n, S_(r));
+    RowArgs ra{};
+    ra.mode = ROW_FFT;
+    ra.nrows = r->nx;
+    ra.ny = r->ny;
+    ra.tw = twy;
+    ra.src = W2;
+    ra.dst = W2;
+    ra.dir = -1;
+    ra.scale = 1.0;
+    launch_row<T>(ra, S_(r));
+    LAUNCHCHK();
+    r->w2_valid = true;
+    r->w2_version = r->est->version;
+    r->w2_sversion = r->s_version;
+  }
+  return 0;
+}
+
+// Fused iterations of PlanarCDI / FresnelCDI (ER and the generic algorithms).
+template <typename T>
+int iterate_fused_pf(nadia_recon* r, int n, double* slots) {
+  const bool two = r->algorithm != NADIA_ER && r->alg[A_PFS] != 0;
+  int rc;
+  if ((rc = fused_prepare<T>(r, two))) return rc;
+  void* twy = twiddles(r->ctx, r->ny, r->prec, true);
+  void* twx = twiddles(r->ctx, r->nx, r->prec, false);
+  if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
+  const int G = r->ny / col_tile<T>(r->nx);
+  // which projection's error get_error() reports: the last one the reference
+  // evaluates (P_F x when a PF or PSF term exists, else P_F P_S x)
+  const int err_field = (r->algorithm == NADIA_ER || r->alg[A_PF] != 0 || r->alg[A_PSF] != 0) ? 0 : 1;
+  ColArgs ca{};
+  ca.mode = r->kind == NADIA_FRESNEL ? COL_FRESNEL : COL_PLANAR;
+  ca.nx = r->nx;
+  ca.ny = r->ny;
+  ca.tw = twx;
+  ca.buf = r->W;
+  ca.field_stride = r->npix;
+  ca.q = r->Q;
+  ca.ill = r->L;
+  ca.sf = 1.0 / std::sqrt((double)r->nx * (double)r->ny);
+  ca.phase_factor = r->phase_factor;
+  ca.xmid = (r->nx - 1) / 2.0;
+  ca.ymid = (r->ny - 1) / 2.0;
+  ca.partials = r->partials;
+  RowArgs ra{};
+  ra.mode = ROW_UPDATE;
+  ra.nrows = r->nx;
+  ra.ny = r->ny;
+  ra.tw = twy;
+  ra.src = r->W;
+  ra.dst = r->W;
+  void* W2 = (char*)r->W + r->npix * r->esz;
+  ra.src2 = two ? W2 : nullptr;
+  ra.dst2 = two ? W2 : nullptr;
+  ra.x = r->est->d;
+  ra.s = r->S;
+  ra.er = r->algorithm == NADIA_ER;
+  ra.a = r->alg[A_PSF];
+  ra.b = r->alg[A_PFS];
+  ra.c = r->alg[A_PS];
+  ra.d = r->alg[A_PF];
+  ra.e = r->alg[A_PI];
+  ra.post = ca.sf;
+  ra.fin_partials = r->partials + 2LL * G * err_field;
+  ra.fin_n = G;
+  std::vector<Ev> evs;
+  for (int k = 0; k < n; ++k) {
+    if ((rc = record_start(r, evs, 0))) return rc;
+    launch_col<T>(ca, two ? 2 : 1, S_(r));
+    if ((rc = record_stop(r, evs))) return rc;
+    LAUNCHCHK();
+    ra.err_out = slots + k;
+    if ((rc = record_start(r, evs, 1))) return rc;
+    launch_row<T>(ra, S_(r));
+    if ((rc = record_stop(r, evs))) return rc;
+    LAUNCHCHK();
+  }
+  r->est->version++;
+  r->w_version = r->est->version;
+  if (two) {
+    r->w2_version = r->est->version;
+    r->w2_sversion = r->s_version;
+  }
+  r->last_err = slots + (n - 1);
+  r->h_err_valid = false;
+  return collect_events(r, evs);
+}
+
+// Fused iterations of FresnelCDI_WF: four passes per iteration.
+template <typename T>
+int iterate_fused_wf(nadia_recon* r, int n, double* slots) {
+  int rc;
+  if ((rc = fused_prepare<T>(r, false))) return rc;
+  void* twy = twiddles(r->ctx, r->ny, r->prec, true);
+  void* twx = twiddles(r->ctx, r->nx, r->prec, false);
+  if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
+  ColArgs c1{};
+  c1.nx = r->nx;
+  c1.ny = r->ny;
+  c1.tw = twx;
+  c1.buf = r->W;
+  c1.field_stride = r->npix;
+  c1.sf = 1.0 / std::sqrt((double)r->nx * (double)r->ny);
+  c1.xmid = (r->nx - 1) / 2.0;
+  c1.ymid = (r->ny - 1) / 2.0;
+  c1.wf_sx = r->wf_sx;
+  c1.wf_sy = r->wf_sy;
+  c1.wf_z12 = r->z12;
+  c1.wf_z23 = r->z23;
+  c1.wf_k = r->wf_k;
+  c1.wf_norm = r->wf_norm;
+  ColArgs c3 = c1;
+  c1.mode = COL_WF1;
+  c3.mode = COL_WF3;
+  RowArgs r2{};
+  r2.mode = ROW_WF2;
+  r2.nrows = r->nx;
+  r2.ny = r->ny;
+  r2.tw = twy;
+  r2.src = r->W;
+  r2.dst = r->W;
+  r2.s = r->S;
+  RowArgs r4 = r2;
+  r4.mode = ROW_WF4;
+  r4.x = r->est->d;
+  r4.q = r->Q;
+  r4.post = c1.sf;
+  r4.partials = r->partials;
+  const int nb = row_blocks<T>(r->nx, r->ny);
+  std::vector<Ev> evs;
+  for (int k = 0; k < n; ++k) {
+    if ((rc = record_start(r, evs, 0))) return rc;
+    launch_col<T>(c1, 1, S_(r));
+    if ((rc = record_stop(r, evs))) return rc;
+    if ((rc = record_start(r, evs, 1))) return rc;
+    launch_row<T>(r2, S_(r));
+    if ((rc = record_stop(r, evs))) return rc;
+    if ((rc = record_start(r, evs, 0))) return rc;
+    launch_col<T>(c3, 1, S_(r));
+    if ((rc = record_stop(r, evs))) return rc;
+    if ((rc = record_start(r, evs, 1))) return rc;
+    launch_row<T>(r4, S_(r));
+    if ((rc = record_stop(r, evs))) return rc;
+    k_final_sum2(r->partials, nb, slots + k, nullptr, S_(r));
+    LAUNCHCHK();
+  }
+  r->est->version++;
+  r->w_version = r->est->version;
+  r->last_err = slots + (n - 1);
+  r->h_err_valid = false;
+  return collect_events(r, evs);
+}
+
+int fetch_error(nadia_recon* r, double* out) {
+  if (r->h_err_valid) {
+    *out = r->h_err_cache;
+    return 0;
+  }
+  if (!r->last_err) {
+    // current_error is never initialised by the reference before the first
+    // projection; report 0
+    *out = 0.0;
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(&r->h_err_cache, r->last_err, sizeof(double), hipMemcpyDeviceToHost, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  r->h_err_valid = true;
+  *out = r->h_err_cache;
+  return 0;
+}
+
+// BaseCDI::update_n_best (NADIA/src/BaseCDI.c++:390-413)
+int update_n_best(nadia_recon* r) {
+  if (r->n_best <= 0) return 0;
+  double err = 0;
+  int rc;
+  if ((rc = fetch_error(r, &err))) return rc;
+  int place = 0;
+  for (; place < r->n_best && err > r->best_err[place]; place++) {}
+  if (place < r->n_best) {
+    void* tp = r->best[r->n_best - 1];
+    HIPCHK(hipMemcpyAsync(tp, r->est->d, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+    for (int i = r->n_best - 1; i > place; i--) {
+      r->best_err[i] = r->best_err[i - 1];
+      r->best[i] = r->best[i - 1];
+    }
+    r->best_err[place] = err;
+    r->best[place] = tp;
+  }
+  return 0;
+}
+
+// set_support on a device real array already in r->rtmp (src/BaseCDI.c++:95-108)
+int support_from_rtmp(nadia_recon* r, int soften) {
+  DISPATCH(r->prec, k_real_max<T>(r->rtmp, r->npix, r->partials, kEwBlocks, S_(r)));
+  k_final_max(r->partials, kEwBlocks, r->scal + 1, S_(r));
+  DISPATCH(r->prec, k_normalise<T>(r->rtmp, r->S, r->scal + 1, r->npix, S_(r)));
+  LAUNCHCHK();
+  if (soften) {
+    // convolve(support, 3, 5): 11x11 window, sigma 3, left unnormalised
+    std::vector<double> g(36);
+    for (int i = 0; i <= 5; i++)
+      for (int j = 0; j <= 5; j++) g[i * 6 + j] = exp(-1 * (i * i + j * j) / (2.0 * 3 * 3));
+    std::vector<float> gf(g.begin(), g.end());
+    if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g.data(), 36 * 8, hipMemcpyHostToDevice, S_(r)));
+    else HIPCHK(hipMemcpyAsync(r->gwin, gf.data(), 36 * 4, hipMemcpyHostToDevice, S_(r)));
+    ConvArgs ca{};
+    ca.src = r->S;
+    ca.src_complex = 0;
+    ca.dst = r->conv;
+    ca.nx = r->nx;
+    ca.ny = r->ny;
+    ca.g = r->gwin;
+    ca.partials = r->partials;
+    int nb = 0;
+    DISPATCH(r->prec, k_conv<T>(ca, 5, &nb, S_(r)));
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(r->S, r->conv, r->npix * r->rsz, hipMemcpyDeviceToDevice, S_(r)));
+    // the host window upload above must complete before `g` goes out of scope
+    HIPCHK(hipStreamSynchronize(S_(r)));
+  }
+  r->s_version++;
+  return 0;
+}
+
+int alloc(void** p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes ? bytes : 16));
+  return 0;
+}
+
+}  // namespace
+
+// =========================== exported C ABI ===============================
+
+extern "C" {
+
+const char* nadia_last_error(void) { return g_err.c_str(); }
+int nadia_version(void) { return 100; }
+
+int nadia_device_count(int* count) {
+  HIPCHK(hipGetDeviceCount(count));
+  return 0;
+}
+
+int nadia_set_device(int device) {
+  HIPCHK(hipSetDevice(device));
+  return 0;
+}
+
+int nadia_size_supported(int n) { return size_supported(n) ? 1 : 0; }
+
+int nadia_synchronize(void) {
+  DeviceCtx* c = dev_ctx(current_device());
+  if (!c) return fail(NADIA_EHIP, "stream creation failed");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ---- fields ---------------------------------------------------------------
+
+int nadia_field_create(int nx, int ny, int precision, nadia_field** out) {
+  if (!out) return fail(NADIA_EINVAL, "null output pointer");
+  if (precision != NADIA_FP32 && precision != NADIA_FP64) return fail(NADIA_EINVAL, "bad precision %d", precision);
+  int rc;
+  if ((rc = check_dims(nx, ny))) return rc;
+  nadia_field* f = new nadia_field{};
+  f->nx = nx;
+  f->ny = ny;
+  f->prec = precision;
+  f->dev = current_device();
+  f->esz = precision == NADIA_FP64 ? 16 : 8;
+  DeviceCtx* c = dev_ctx(f->dev);
+  if (!c) {
+    delete f;
+    return fail(NADIA_EHIP, "stream creation failed");
+  }
+  if (hipMalloc(&f->d, (size_t)nx * ny * f->esz) != hipSuccess) {
+    delete f;
+    return fail(NADIA_EHIP, "hipMalloc of %dx%d field failed", nx, ny);
+  }
+  HIPCHK(hipMemsetAsync(f->d, 0, (size_t)nx * ny * f->esz, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = f;
+  return 0;
+}
+
+int nadia_field_destroy(nadia_field* f) {
+  if (!f) return 0;
+  DeviceCtx* c = dev_ctx(f->dev);
+  if (c) (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(f->d);
+  delete f;
+  return 0;
+}
+
+int nadia_field_dims(const nadia_field* f, int* nx, int* ny, int* precision) {
+  if (!f) return fail(NADIA_EINVAL, "null field");
+  if (nx) *nx = f->nx;
+  if (ny) *ny = f->ny;
+  if (precision) *precision = f->prec;
+  return 0;
+}
+
+int nadia_field_upload(nadia_field* f, const void* host) {
+  if (!f || !host) return fail(NADIA_EINVAL, "null argument");
+  DeviceCtx* c = dev_ctx(f->dev);
+  HIPCHK(hipMemcpyAsync(f->d, host, (size_t)f->nx * f->ny * f->esz, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  f->version++;
+  return 0;
+}
+
+int nadia_field_download(const nadia_field* f, void* host) {
+  if (!f || !host) return fail(NADIA_EINVAL, "null argument");
+  DeviceCtx* c = dev_ctx(f->dev);
+  HIPCHK(hipMemcpyAsync(host, f->d, (size_t)f->nx * f->ny * f->esz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int nadia_field_copy(nadia_field* dst, const nadia_field* src) {
+  if (!dst || !src) return fail(NADIA_EINVAL, "null argument");
+  if (dst->nx != src->nx || dst->ny != src->ny || dst->prec != src->prec)
+    return fail(NADIA_EINVAL, "copy between fields of different dimensions or precision");
+  DeviceCtx* c = dev_ctx(dst->dev);
+  HIPCHK(hipMemcpyAsync(dst->d, src->d, (size_t)dst->nx * dst->ny * dst->esz, hipMemcpyDeviceToDevice, c->stream));
+  dst->version++;
+  return 0;
+}
+
+int nadia_field_fft(nadia_field* f, int direction) {
+  if (!f || (direction != -1 && direction != 1)) return fail(NADIA_EINVAL, "bad argument");
+  int rc = fft2d(dev_ctx(f->dev), f->d, f->nx, f->ny, f->prec, direction);
+  f->version++;
+  return rc;
+}
+
+int nadia_field_invert(nadia_field* f, int scale) {
+  if (!f) return fail(NADIA_EINVAL, "null field");
+  DeviceCtx* c = dev_ctx(f->dev);
+  DISPATCH(f->prec, k_invert<T>(f->d, f->nx, f->ny, scale, c->stream));
+  LAUNCHCHK();
+  f->version++;
+  return 0;
+}
+
+int nadia_field_scale(nadia_field* f, double factor) {
+  if (!f) return fail(NADIA_EINVAL, "null field");
+  DeviceCtx* c = dev_ctx(f->dev);
+  DISPATCH(f->prec, k_scale<T>(f->d, factor, (long long)f->nx * f->ny, c->stream));
+  LAUNCHCHK();
+  f->version++;
+  return 0;
+}
+
+int nadia_field_add(nadia_field* f, const nadia_field* other, double scale) {
+  if (!f || !other) return fail(NADIA_EINVAL, "null field");
+  if (f->nx != other->nx || f->ny != other->ny || f->prec != other->prec)
+    return fail(NADIA_EINVAL, "add: dimensions of the input do not match");
+  DeviceCtx* c = dev_ctx(f->dev);
+  DISPATCH(f->prec, k_fresnel_add<T>(f->d, other->d, scale, (long long)f->nx * f->ny, c->stream));
+  LAUNCHCHK();
+  f->version++;
+  return 0;
+}
+
+int nadia_field_norm(const nadia_field* f, double* out) {
+  if (!f || !out) return fail(NADIA_EINVAL, "null argument");
+  DeviceCtx* c = dev_ctx(f->dev);
+  double* part = nullptr;
+  HIPCHK(hipMalloc(&part, (kEwBlocks + 1) * sizeof(double)));
+  DISPATCH(f->prec, k_norm2<T>(f->d, (long long)f->nx * f->ny, part, kEwBlocks, c->stream));
+  k_final_sum1(part, kEwBlocks, part + kEwBlocks, c->stream);
+  double s = 0;
+  HIPCHK(hipMemcpyAsync(&s, part + kEwBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(part);
+  *out = std::sqrt(s);
+  return 0;
+}
+
+int nadia_field_magnitude(const nadia_field* f, void* host_real) {
+  if (!f || !host_real) return fail(NADIA_EINVAL, "null argument");
+  DeviceCtx* c = dev_ctx(f->dev);
+  void* tmp = nullptr;
+  size_t rsz = f->esz / 2;
+  HIPCHK(hipMalloc(&tmp, (size_t)f->nx * f->ny * rsz));
+  DISPATCH(f->prec, k_magnitude<T>(f->d, tmp, (long long)f->nx * f->ny, c->stream));
+  HIPCHK(hipMemcpyAsync(host_real, tmp, (size_t)f->nx * f->ny * rsz, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(tmp);
+  return 0;
+}
+
+// ---- reconstructions --------------------------------------------------------
+
+int nadia_recon_set_algorithm(nadia_recon* r, int alg);
+
+int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out) {
+  if (!est || !out) return fail(NADIA_EINVAL, "null argument");
+  if (kind < NADIA_PLANAR || kind > NADIA_FRESNEL_WF) return fail(NADIA_EINVAL, "bad kind %d", kind);
+  if (n_best < 0) return fail(NADIA_EINVAL, "negative n_best");
+  if ((est->nx & 1) || (est->ny & 1)) return fail(NADIA_ESIZE, "odd dimensions are not supported");
+  nadia_recon* r = new nadia_recon{};
+  r->est = est;
+  r->kind = kind;
+  r->prec = est->prec;
+  r->nx = est->nx;
+  r->ny = est->ny;
+  r->dev = est->dev;
+  r->n_best = n_best;
+  r->ctx = dev_ctx(r->dev);
+  if (!r->ctx) {
+    delete r;
+    return fail(NADIA_EHIP, "stream creation failed");
+  }
+  r->esz = est->esz;
+  r->rsz = est->esz / 2;
+  r->npix = (long long)r->nx * r->ny;
+  r->ring_cap = 4096;
+  r->path = NADIA_PATH_FUSED;
+  r->beta = 0.9;
+  const size_t cb = r->npix * r->esz, rb = r->npix * r->rsz;
+  int rc = 0;
+  if ((rc = alloc(&r->S, rb)) || (rc = alloc(&r->Q, rb)) || (rc = alloc(&r->W, 2 * cb)) ||
+      (rc = alloc(&r->conv, rb)) || (rc = alloc(&r->rtmp, rb)) || (rc = alloc(&r->gwin, 64 * 8)) ||
+      (rc = alloc((void**)&r->partials, sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16))) ||
+      (rc = alloc((void**)&r->errs, sizeof(double) * (r->ring_cap + 8))) ||
+      (rc = alloc((void**)&r->scal, sizeof(double) * 16))) {
+    delete r;
+    return rc;
+  }
+  (void)hipMemsetAsync(r->S, 0, rb, S_(r));
+  (void)hipMemsetAsync(r->Q, 0, rb, S_(r));
+  for (int i = 0; i < n_best; i++) {
+    void* p = nullptr;
+    if (hipMalloc(&p, cb) != hipSuccess) {
+      delete r;
+      return fail(NADIA_EHIP, "hipMalloc failed for n_best snapshot");
+    }
+    (void)hipMemsetAsync(p, 0, cb, S_(r));
+    r->best.push_back(p);
+    r->best_err.push_back(1.0);
+  }
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  // BaseCDI::BaseCDI: algorithm = CUSTOM; set_algorithm(ER) (src/BaseCDI.c++:63-64)
+  r->algorithm = NADIA_CUSTOM;
+  nadia_recon_set_algorithm(r, NADIA_ER);
+  *out = r;
+  return 0;
+}
+
+int nadia_recon_destroy(nadia_recon* r) {
+  if (!r) return 0;
+  (void)hipStreamSynchronize(S_(r));
+  void* ptrs[] = {r->S, r->Q, r->beam, r->W, r->t[0], r->t[1], r->t[2], r->t[3], r->conv, r->rtmp,
+                  r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (void* p : r->best) (void)hipFree(p);
+  delete r;
+  return 0;
+}
+
+int nadia_recon_set_path(nadia_recon* r, int path) {
+  if (!r || (path != NADIA_PATH_FUSED && path != NADIA_PATH_PLAIN)) return fail(NADIA_EINVAL, "bad argument");
+  r->path = path;
+  r->w_valid = r->w2_valid = false;
+  return 0;
+}
+
+int nadia_recon_set_support(nadia_recon* r, const void* host, int soften) {
+  if (!r || !host) return fail(NADIA_EINVAL, "null argument");
+  HIPCHK(hipMemcpyAsync(r->rtmp, host, r->npix * r->rsz, hipMemcpyHostToDevice, S_(r)));
+  int rc = support_from_rtmp(r, soften);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  return 0;
+}
+
+int nadia_recon_set_intensity(nadia_recon* r, const void* host) {
+  if (!r || !host) return fail(NADIA_EINVAL, "null argument");
+  HIPCHK(hipMemcpyAsync(r->rtmp, host, r->npix * r->rsz, hipMemcpyHostToDevice, S_(r)));
+  DISPATCH(r->prec, k_make_q<T>(r->rtmp, r->beam, r->has_beam, r->Q, r->npix, S_(r)));
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  r->has_intensity = 1;
+  return 0;
+}
+
+int nadia_recon_set_beam_stop(nadia_recon* r, const void* host) {
+  if (!r || !host) return fail(NADIA_EINVAL, "null argument");
+  if (!r->beam) {
+    int rc = alloc(&r->beam, r->npix * r->rsz);
+    if (rc) return rc;
+  }
+  HIPCHK(hipMemcpyAsync(r->beam, host, r->npix * r->rsz, hipMemcpyHostToDevice, S_(r)));
+  r->has_beam = 1;
+  // re-mask sqrt(I) already on the device
+  DISPATCH(r->prec, k_make_q<T>(nullptr, r->beam, 1, r->Q, r->npix, S_(r)));
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  return 0;
+}
+
+int nadia_recon_set_custom_algorithm(nadia_recon* r, const double m[10]) {
+  if (!r || !m) return fail(NADIA_EINVAL, "null argument");
+  if (r->kind == NADIA_FRESNEL_WF) return 0;  // hidden no-op, FresnelCDI_WF.h:176-181
+  // src/BaseCDI.c++:245-249
+  r->alg[A_PSF] = m[0] + m[1] + m[2] + m[3];
+  r->alg[A_PFS] = -m[0] + m[4] + m[5] + m[6];
+  r->alg[A_PS] = -m[2] - m[5] - m[7] + m[9];
+  r->alg[A_PF] = -m[1] - m[4] + m[7] + m[8];
+  r->alg[A_PI] = -m[3] - m[6] - m[8] - m[9];
+  // reallocate_temp_complex_memory (src/BaseCDI.c++:257-275)
+  for (int n = 0; n < 4; n++) {
+    if (r->alg[n] == 0 && r->t[n]) {
+      (void)hipStreamSynchronize(S_(r));
+      (void)hipFree(r->t[n]);
+      r->t[n] = nullptr;
+    }
+    if (r->alg[n] != 0 && !r->t[n]) {
+      int rc = alloc(&r->t[n], r->npix * r->esz);
+      if (rc) return rc;
+    }
+  }
+  r->algorithm = NADIA_CUSTOM;
+  r->w2_valid = false;
+  return 0;
+}
+
+int nadia_recon_set_algorithm(nadia_recon* r, int alg) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  // FresnelCDI_WF hides set_algorithm (FresnelCDI_WF.h:147-150); only the
+  // constructor's BaseCDI call takes effect
+  if (r->kind == NADIA_FRESNEL_WF && r->algorithm != NADIA_CUSTOM) return 0;
+  if (r->algorithm == alg) return 0;  // src/BaseCDI.c++:194-195
+  const double b = r->beta;
+  double m[10] = {0};
+  switch (alg) {
+    case NADIA_ER: break;
+    case NADIA_BIO: m[1] = b; break;
+    case NADIA_BOO: m[1] = b; m[8] = 1; break;
+    case NADIA_HIO: m[1] = b; m[2] = 1; break;
+    case NADIA_DM: m[0] = b; m[2] = 1; m[4] = 1; break;
+    case NADIA_SF: m[1] = 1; m[3] = 1; break;
+    case NADIA_ASR: m[1] = 1; m[2] = 1; break;
+    case NADIA_HPR: m[1] = b; m[2] = 1; break;
+    case NADIA_RAAR: m[1] = 1; m[2] = b; m[8] = 1 - b; break;
+    default:
+      fprintf(stdout, "Algorithm unknown\n");
+      r->algorithm = alg;
+      return 0;
+  }
+  int saved_kind = r->kind;
+  r->kind = NADIA_PLANAR;  // let set_custom_algorithm run for every kind here
+  int rc = nadia_recon_set_custom_algorithm(r, m);
+  r->kind = saved_kind;
+  if (rc) return rc;
+  r->algorithm = alg;
+  return 0;
+}
+
+int nadia_recon_set_relaxation_parameter(nadia_recon* r, double beta) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  if (r->kind == NADIA_FRESNEL_WF) return 0;  // FresnelCDI_WF.h:168-172
+  r->beta = beta;
+  return nadia_recon_set_algorithm(r, r->algorithm);  // include/BaseCDI.h:264-267
+}
+
+int nadia_recon_get_coefficients(const nadia_recon* r, double out5[5], int* algorithm) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  if (out5)
+    for (int i = 0; i < 5; i++) out5[i] = r->alg[i];
+  if (algorithm) *algorithm = r->algorithm;
+  return 0;
+}
+
+int nadia_recon_iterate(nadia_recon* r, int n, double* errors) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  if (n <= 0) return 0;
+  if (!r->has_intensity) return fail(NADIA_ESTATE, "intensity not set");
+  int rc;
+  if ((rc = need_setup(r))) return rc;
+  const bool nb = r->n_best > 0 && r->kind != NADIA_FRESNEL_WF;
+  int done = 0;
+  while (done < n) {
+    int chunk = n - done;
+    if (chunk > r->ring_cap) chunk = r->ring_cap;
+    if (nb) chunk = 1;  // the n-best bookkeeping needs each error on the host
+    if (fused_ok(r)) {
+      if (r->kind == NADIA_FRESNEL_WF) DISPATCH(r->prec, rc = iterate_fused_wf<T>(r, chunk, r->errs));
+      else DISPATCH(r->prec, rc = iterate_fused_pf<T>(r, chunk, r->errs));
+      if (rc) return rc;
+    } else {
+      for (int k = 0; k < chunk; ++k) {
+        if ((rc = iterate_plain(r, r->errs + k))) return rc;
+      }
+      r->est->version++;
+    }
+    if (nb && (rc = update_n_best(r))) return rc;
+    if (errors) {
+      if (has_projection(r)) {
+        HIPCHK(hipMemcpyAsync(errors + done, r->errs, chunk * sizeof(double), hipMemcpyDeviceToHost, S_(r)));
+      } else {
+        // a projection-free custom algorithm leaves get_error() unchanged
+        double v = 0;
+        if ((rc = fetch_error(r, &v))) return rc;
+        for (int k = 0; k < chunk; ++k) errors[done + k] = v;
+      }
+    }
+    done += chunk;
+    if (done < n || errors) HIPCHK(hipStreamSynchronize(S_(r)));
+  }
+  // keep get_error() valid after the ring is reused by the next call
+  if (r->last_err && r->last_err != r->scal + 8) {
+    HIPCHK(hipMemcpyAsync(r->scal + 8, r->last_err, sizeof(double), hipMemcpyDeviceToDevice, S_(r)));
+    r->last_err = r->scal + 8;
+  }
+  return 0;
+}
+
+int nadia_recon_get_error(nadia_recon* r, double* out) {
+  if (!r || !out) return fail(NADIA_EINVAL, "null argument");
+  return fetch_error(r, out);
+}
+
+int nadia_recon_apply_shrinkwrap(nadia_recon* r, double gauss_width, double threshold) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  // window g(i,j) = exp(-(i^2+j^2)/(2 sigma^2)), i,j in [0,4], stored in the
+  // field's real type (BaseCDI::convolve, src/BaseCDI.c++:441-447)
+  double g[25];
+  for (int i = 0; i <= 4; i++)
+    for (int j = 0; j <= 4; j++) {
+      double denom = 2.0 * gauss_width * gauss_width;
+      g[i * 5 + j] = exp(-1 * (i * i + j * j) / denom);
+    }
+  float gf[25];
+  for (int i = 0; i < 25; i++) gf[i] = (float)g[i];
+  if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g, sizeof(g), hipMemcpyHostToDevice, S_(r)));
+  else HIPCHK(hipMemcpyAsync(r->gwin, gf, sizeof(gf), hipMemcpyHostToDevice, S_(r)));
+  ConvArgs ca{};
+  ca.src = r->est->d;
+  ca.src_complex = 1;
+  ca.dst = r->conv;
+  ca.nx = r->nx;
+  ca.ny = r->ny;
+  ca.g = r->gwin;
+  ca.partials = r->partials;
+  int nb = 0;
+  DISPATCH(r->prec, k_conv<T>(ca, 4, &nb, S_(r)));
+  k_final_max(r->partials, nb, r->scal, S_(r));
+  DISPATCH(r->prec, k_threshold<T>(r->conv, r->rtmp, r->nx, r->ny, r->scal, threshold, S_(r)));
+  LAUNCHCHK();
+  int rc = support_from_rtmp(r, 0);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(S_(r)));  // g / gf live on this stack frame
+  return 0;
+}
+
+int nadia_recon_get_support(const nadia_recon* r, void* host) {
+  if (!r || !host) return fail(NADIA_EINVAL, "null argument");
+  HIPCHK(hipMemcpyAsync(host, r->S, r->npix * r->rsz, hipMemcpyDeviceToHost, r->ctx->stream));
+  HIPCHK(hipStreamSynchronize(r->ctx->stream));
+  return 0;
+}
+
+int nadia_recon_initialise_estimate(nadia_recon* r, int seed) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  int rc;
+  const long long n = r->npix;
+  std::vector<double> s(n), q(n);
+  // download support and sqrt(I) in the field's real type
+  {
+    std::vector<char> bs(n * r->rsz), bq(n * r->rsz);
+    HIPCHK(hipMemcpyAsync(bs.data(), r->S, n * r->rsz, hipMemcpyDeviceToHost, S_(r)));
+    HIPCHK(hipMemcpyAsync(bq.data(), r->Q, n * r->rsz, hipMemcpyDeviceToHost, S_(r)));
+    HIPCHK(hipStreamSynchronize(S_(r)));
+    for (long long p = 0; p < n; p++) {
+      if (r->prec == NADIA_FP64) {
+        s[p] = ((double*)bs.data())[p];
+        q[p] = std::fabs(((double*)bq.data())[p]);
+      } else {
+        s[p] = ((float*)bs.data())[p];
+        q[p] = std::fabs(((float*)bq.data())[p]);
+      }
+    }
+  }
+  std::vector<double> x(2 * n, 0.0);
+  if (r->kind == NADIA_PLANAR) {
+    // PlanarCDI::initialise_estimate (src/PlanarCDI.c++:39-62)
+    srand(seed);
+    double qmax = n ? q[0] : 0;
+    for (long long p = 0; p < n; p++)
+      if (q[p] > qmax) qmax = q[p];
+    if (r->prec == NADIA_FP32) qmax = (float)qmax;
+    const int max_value = (int)qmax;
+    for (long long p = 0; p < n; p++) {
+      if (!s[p]) continue;
+      // max_value*rand() is an int product in the reference; it wraps
+      const int pr = (int)((unsigned)max_value * (unsigned)rand());
+      const double re = s[p] * (pr / (double)RAND_MAX);
+      const int pi = (int)((unsigned)max_value * (unsigned)rand());
+      const double im = s[p] * (pi / (double)RAND_MAX);
+      x[2 * p] = re;
+      x[2 * p + 1] = im;
+    }
+  } else if (r->kind == NADIA_FRESNEL) {
+    // FresnelCDI::initialise_estimate (src/FresnelCDI.c++:152-201)
+    if ((rc = need_setup(r))) return rc;
+    srand(seed);
+    std::vector<double> tr(2 * n, 0.0);
+    for (long long p = 0; p < n; p++) tr[2 * p] = 1 - 0.2 * rand() / ((double)RAND_MAX);
+    void* tbuf = r->t[0] ? r->t[0] : nullptr;
+    bool own = false;
+    if (!tbuf) {
+      if ((rc = alloc(&tbuf, n * r->esz))) return rc;
+      own = true;
+    }
+    if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(tbuf, tr.data(), n * 16, hipMemcpyHostToDevice, S_(r)));
+    else {
+      std::vector<float> f(tr.begin(), tr.end());
+      HIPCHK(hipMemcpyAsync(tbuf, f.data(), n * 8, hipMemcpyHostToDevice, S_(r)));
+      HIPCHK(hipStreamSynchronize(S_(r)));
+    }
+    DISPATCH(r->prec, k_esw_from_trans<T>(tbuf, r->Ls, r->est->d, r->nx, r->ny, S_(r)));
+    if ((rc = op_support(r, r->est->d))) return rc;
+    HIPCHK(hipStreamSynchronize(S_(r)));
+    if (own) (void)hipFree(tbuf);
+    r->est->version++;
+    return 0;
+  } else {
+    // FresnelCDI_WF::initialise_estimate (src/FresnelCDI_WF.c++:99-117)
+    srand(seed);
+    for (long long p = 0; p < n; p++)
+      if (s[p]) x[2 * p] = q[p];
+  }
+  if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->est->d, x.data(), n * 16, hipMemcpyHostToDevice, S_(r)));
+  else {
+    std::vector<float> f(x.begin(), x.end());
+    HIPCHK(hipMemcpyAsync(r->est->d, f.data(), n * 8, hipMemcpyHostToDevice, S_(r)));
+    HIPCHK(hipStreamSynchronize(S_(r)));
+  }
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  r->est->version++;
+  return 0;
+}
+
+static int same_shape(nadia_recon* r, nadia_field* c) {
+  if (!r || !c) return fail(NADIA_EINVAL, "null argument");
+  if (c->nx != r->nx || c->ny != r->ny || c->prec != r->prec)
+    return fail(NADIA_EINVAL, "field does not match the reconstruction's size or precision");
+  return 0;
+}
+
+int nadia_recon_apply_support(nadia_recon* r, nadia_field* c) {
+  int rc;
+  if ((rc = same_shape(r, c))) return rc;
+  c->version++;
+  return op_support(r, c->d);
+}
+
+int nadia_recon_project_intensity(nadia_recon* r, nadia_field* c) {
+  int rc;
+  if ((rc = same_shape(r, c))) return rc;
+  c->version++;
+  return op_project(r, c->d, r->scal + 9);
+}
+
+int nadia_recon_scale_intensity(nadia_recon* r, nadia_field* c) {
+  int rc;
+  if ((rc = same_shape(r, c))) return rc;
+  c->version++;
+  return op_scale_intensity(r, c->d, r->scal + 9);
+}
+
+int nadia_recon_propagate_to_detector(nadia_recon* r, nadia_field* c) {
+  int rc;
+  if ((rc = same_shape(r, c))) return rc;
+  c->version++;
+  return op_to_detector(r, c->d);
+}
+
+int nadia_recon_propagate_from_detector(nadia_recon* r, nadia_field* c) {
+  int rc;
+  if ((rc = same_shape(r, c))) return rc;
+  c->version++;
+  return op_from_detector(r, c->d);
+}
+
+int nadia_recon_get_best_result(nadia_recon* r, int index, double* error, void* host) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  if (index < 0 || index >= r->n_best) {
+    fprintf(stdout, "index out of bounds in BaseCDI::get_best_result\n");
+    return fail(NADIA_EINVAL, "index out of bounds in get_best_result");
+  }
+  if (error) *error = r->best_err[index];
+  if (host) {
+    HIPCHK(hipMemcpyAsync(host, r->best[index], r->npix * r->esz, hipMemcpyDeviceToHost, S_(r)));
+    HIPCHK(hipStreamSynchronize(S_(r)));
+  }
+  return 0;
+}
+
+int nadia_recon_reset_best(nadia_recon* r) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  for (int i = 0; i < r->n_best; i++) r->best_err[i] = 1;
+  return 0;
+}
+
+int nadia_recon_get_intensity_autocorrelation(nadia_recon* r, void* host_real) {
+  if (!r || !host_real) return fail(NADIA_EINVAL, "null argument");
+  void* tmp = nullptr;
+  int rc;
+  if ((rc = alloc(&tmp, r->npix * r->esz))) return rc;
+  DISPATCH(r->prec, k_autocorr_prep<T>(r->Q, tmp, r->npix, S_(r)));
+  if ((rc = fft2d(r->ctx, tmp, r->nx, r->ny, r->prec, +1))) return rc;
+  if ((rc = op_invert(r, tmp))) return rc;
+  DISPATCH(r->prec, k_magnitude<T>(tmp, r->conv, r->npix, S_(r)));
+  HIPCHK(hipMemcpyAsync(host_real, r->conv, r->npix * r->rsz, hipMemcpyDeviceToHost, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  (void)hipFree(tmp);
+  return 0;
+}
+
+// ---- FresnelCDI ---------------------------------------------------------
+
+int nadia_fresnel_set_experimental_parameters(nadia_recon* r, double wl, double zfd, double zfs, double pixel) {
+  if (!r || r->kind != NADIA_FRESNEL) return fail(NADIA_EINVAL, "not a FresnelCDI reconstruction");
+  if (!r->L) return fail(NADIA_ESTATE, "white field not set (nadia_fresnel_setup)");
+  // src/FresnelCDI.c++:57-95
+  r->wl = wl;
+  r->pixel = pixel;
+  r->zfd = zfd;
+  r->zfs = zfs;
+  const double zsd = zfd - zfs;
+  r->phase_factor = pixel * pixel * M_PI / (wl) * ((1 / zfd) - (1 / zsd));
+  r->fresnel_ready = true;
+  int rc;
+  if (!r->Ls && (rc = alloc(&r->Ls, r->npix * r->esz))) return rc;
+  HIPCHK(hipMemcpyAsync(r->Ls, r->L, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+  if ((rc = op_from_detector(r, r->Ls))) return rc;
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  return 0;
+}
+
+int nadia_fresnel_setup(nadia_recon* r, const void* white_field, double wl, double zfd, double zfs, double pixel,
+                        double normalisation) {
+  if (!r || r->kind != NADIA_FRESNEL) return fail(NADIA_EINVAL, "not a FresnelCDI reconstruction");
+  if (!white_field) return fail(NADIA_EINVAL, "null white field");
+  int rc;
+  if (!r->L && (rc = alloc(&r->L, r->npix * r->esz))) return rc;
+  // src/FresnelCDI.c++:20-44: illumination = white_field * norm
+  r->norm = normalisation;
+  HIPCHK(hipMemcpyAsync(r->L, white_field, r->npix * r->esz, hipMemcpyHostToDevice, S_(r)));
+  DISPATCH(r->prec, k_scale<T>(r->L, normalisation, r->npix, S_(r)));
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  return nadia_fresnel_set_experimental_parameters(r, wl, zfd, zfs, pixel);
+}
+
+int nadia_fresnel_set_norm(nadia_recon* r, double nn) {
+  if (!r || r->kind != NADIA_FRESNEL || !r->L) return fail(NADIA_ESTATE, "FresnelCDI not set up");
+  // src/FresnelCDI.c++:139-148
+  const double old = r->norm;
+  r->norm = nn;
+  DISPATCH(r->prec, k_scale<T>(r->L, nn / old, r->npix, S_(r)));
+  if (r->Ls) DISPATCH(r->prec, k_scale<T>(r->Ls, nn / old, r->npix, S_(r)));
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  return 0;
+}
+
+int nadia_fresnel_set_normalisation(nadia_recon* r, double nn) {
+  if (!r || r->kind != NADIA_FRESNEL) return fail(NADIA_EINVAL, "not a FresnelCDI reconstruction");
+  r->norm = nn;  // include/FresnelCDI.h:154-156 stores the value only
+  return 0;
+}
+
+int nadia_fresnel_auto_set_norm(nadia_recon* r) {
+  if (!r || r->kind != NADIA_FRESNEL || !r->L) return fail(NADIA_ESTATE, "FresnelCDI not set up");
+  // src/FresnelCDI.c++:129-137: sum sqrt(I) / sum |white field|, sums in the
+  // field's real type like Real_2D::get_sum
+  const long long n = r->npix;
+  std::vector<char> bl(n * r->esz), bq(n * r->rsz);
+  HIPCHK(hipMemcpyAsync(bl.data(), r->L, n * r->esz, hipMemcpyDeviceToHost, S_(r)));
+  HIPCHK(hipMemcpyAsync(bq.data(), r->Q, n * r->rsz, hipMemcpyDeviceToHost, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  double wf, in;
+  if (r->prec == NADIA_FP64) {
+    double a = 0, b = 0;
+    const double* L = (const double*)bl.data();
+    const double* q = (const double*)bq.data();
+    for (long long p = 0; p < n; p++) {
+      a += std::sqrt(L[2 * p] * L[2 * p] + L[2 * p + 1] * L[2 * p + 1]);
+      b += std::fabs(q[p]);
+    }
+    wf = a;
+    in = b;
+  } else {
+    float a = 0, b = 0;
+    const float* L = (const float*)bl.data();
+    const float* q = (const float*)bq.data();
+    for (long long p = 0; p < n; p++) {
+      a += std::sqrt(L[2 * p] * L[2 * p] + L[2 * p + 1] * L[2 * p + 1]);
+      b += std::fabs(q[p]);
+    }
+    wf = a;
+    in = b;
+  }
+  return nadia_fresnel_set_norm(r, in / wf);
+}
+
+int nadia_fresnel_get_illumination_at_sample(nadia_recon* r, void* host) {
+  if (!r || r->kind != NADIA_FRESNEL || !r->Ls) return fail(NADIA_ESTATE, "FresnelCDI not set up");
+  HIPCHK(hipMemcpyAsync(host, r->Ls, r->npix * r->esz, hipMemcpyDeviceToHost, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  return 0;
+}
+
+int nadia_fresnel_get_transmission_function(nadia_recon* r, const nadia_field* esw, void* host) {
+  if (!r || r->kind != NADIA_FRESNEL || !r->Ls) return fail(NADIA_ESTATE, "FresnelCDI not set up");
+  if (!host) return fail(NADIA_EINVAL, "null output");
+  const nadia_field* e = esw ? esw : r->est;
+  void* tmp = nullptr;
+  int rc;
+  if ((rc = alloc(&tmp, r->npix * r->esz))) return rc;
+  DISPATCH(r->prec, k_trans_from_esw<T>(e->d, r->Ls, tmp, r->npix, S_(r)));
+  HIPCHK(hipMemcpyAsync(host, tmp, r->npix * r->esz, hipMemcpyDeviceToHost, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  (void)hipFree(tmp);
+  return 0;
+}
+
+int nadia_fresnel_set_transmission_function(nadia_recon* r, const void* host, nadia_field* esw) {
+  if (!r || r->kind != NADIA_FRESNEL || !r->Ls) return fail(NADIA_ESTATE, "FresnelCDI not set up");
+  if (!host) return fail(NADIA_EINVAL, "null input");
+  nadia_field* e = esw ? esw : r->est;
+  void* tmp = nullptr;
+  int rc;
+  if ((rc = alloc(&tmp, r->npix * r->esz))) return rc;
+  HIPCHK(hipMemcpyAsync(tmp, host, r->npix * r->esz, hipMemcpyHostToDevice, S_(r)));
+  DISPATCH(r->prec, k_esw_from_trans<T>(tmp, r->Ls, e->d, r->nx, r->ny, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  (void)hipFree(tmp);
+  e->version++;
+  return 0;
+}
+
+// ---- FresnelCDI_WF ----------------------------------------------------
+
+int nadia_wf_setup(nadia_recon* r, double wl, double z12, double z23, double pixel) {
+  if (!r || r->kind != NADIA_FRESNEL_WF) return fail(NADIA_EINVAL, "not a FresnelCDI_WF reconstruction");
+  // src/FresnelCDI_WF.c++:36-58
+  r->wl = wl;
+  r->z12 = z12;
+  r->z23 = z23;
+  r->pixel = pixel;
+  r->wf_sx = wl * z23 / (pixel * r->nx);
+  r->wf_sy = wl * z23 / (pixel * r->ny);
+  r->wf_k = M_PI / wl;
+  r->wf_norm = 1 / (std::sqrt((double)(r->nx * r->ny)));
+  r->wf_ready = true;
+  return 0;
+}
+
+int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size) {
+  if (!r || r->kind != NADIA_FRESNEL_WF || !r->wf_ready) return fail(NADIA_ESTATE, "FresnelCDI_WF not set up");
+  // src/FresnelCDI_WF.c++:177-196: integer centre (nx-1)/2
+  const double centre = (double)((r->nx - 1) / 2);
+  const double dz = (r->z12 / r->z23) * r->pixel;
+  DISPATCH(r->prec, k_wf_disk<T>(r->S, r->nx, r->ny, centre, dz, size * z_diameter / 2.0, S_(r)));
+  LAUNCHCHK();
+  // convolve(support, 3, 5), left unnormalised
+  HIPCHK(hipMemcpyAsync(r->rtmp, r->S, r->npix * r->rsz, hipMemcpyDeviceToDevice, S_(r)));
+  std::vector<double> g(36);
+  for (int i = 0; i <= 5; i++)
+    for (int j = 0; j <= 5; j++) g[i * 6 + j] = exp(-1 * (i * i + j * j) / (2.0 * 3 * 3));
+  std::vector<float> gf(g.begin(), g.end());
+  if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g.data(), 36 * 8, hipMemcpyHostToDevice, S_(r)));
+  else HIPCHK(hipMemcpyAsync(r->gwin, gf.data(), 36 * 4, hipMemcpyHostToDevice, S_(r)));
+  ConvArgs ca{};
+  ca.src = r->rtmp;
+  ca.src_complex = 0;
+  ca.dst = r->S;
+  ca.nx = r->nx;
+  ca.ny = r->ny;
+  ca.g = r->gwin;
+  ca.partials = r->partials;
+  int nb = 0;
+  DISPATCH(r->prec, k_conv<T>(ca, 5, &nb, S_(r)));
+  LAUNCHCHK();
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  r->s_version++;
+  return 0;
+}
+
+// ---- measurement ---------------------------------------------------------
+
+int nadia_recon_set_timing(nadia_recon* r, int on) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  r->timing = on != 0;
+  r->pass_ms[0] = r->pass_ms[1] = 0;
+  r->pass_n[0] = r->pass_n[1] = 0;
+  return 0;
+}
+
+int nadia_recon_pass_times(nadia_recon* r, double ms[2], long long count[2]) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  for (int i = 0; i < 2; i++) {
+    if (ms) ms[i] = r->pass_ms[i];
+    if (count) count[i] = r->pass_n[i];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,176 @@
+// fft_device.h -- device building blocks: complex helpers, radix-2/3/4/8
+// butterflies and the register/LDS Stockham stage loop (see plan.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "plan.h"
+
+namespace nadia {
+
+template <typename T> struct Vec2;
+template <> struct Vec2<float> { using type = float2; };
+template <> struct Vec2<double> { using type = double2; };
+template <typename T> using c2_t = typename Vec2<T>::type;
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * (DIR * i)
+template <int DIR, typename C> __device__ __forceinline__ C cmul_i(C a) {
+  return DIR < 0 ? C{a.y, -a.x} : C{-a.y, a.x};
+}
+template <typename C, typename T> __device__ __forceinline__ C cscale(C a, T s) { return C{a.x * s, a.y * s}; }
+
+// Size-R DFT in place, exponent sign DIR (-1 forward, +1 backward).
+template <int R, int DIR, typename T>
+__device__ __forceinline__ void dft(c2_t<T> (&a)[R]) {
+  using C = c2_t<T>;
+  if constexpr (R == 2) {
+    C t = a[0];
+    a[0] = cadd(t, a[1]);
+    a[1] = csub(t, a[1]);
+  } else if constexpr (R == 3) {
+    const T h = (T)0.86602540378443864676372317075294;  // sqrt(3)/2
+    C s = cadd(a[1], a[2]);
+    C d = csub(a[1], a[2]);
+    C m = C{a[0].x - (T)0.5 * s.x, a[0].y - (T)0.5 * s.y};
+    a[0] = cadd(a[0], s);
+    C u = cmul_i<DIR>(d);
+    u = cscale(u, h);
+    a[1] = cadd(m, u);
+    a[2] = csub(m, u);
+  } else if constexpr (R == 4) {
+    C t0 = cadd(a[0], a[2]), t1 = csub(a[0], a[2]);
+    C t2 = cadd(a[1], a[3]), t3 = csub(a[1], a[3]);
+    C u = cmul_i<DIR>(t3);
+    a[0] = cadd(t0, t2);
+    a[2] = csub(t0, t2);
+    a[1] = cadd(t1, u);
+    a[3] = csub(t1, u);
+  } else if constexpr (R == 8) {
+    const T h = (T)0.70710678118654752440084436210485;  // sqrt(2)/2
+    // even / odd radix-4 halves
+    C e[4] = {a[0], a[2], a[4], a[6]};
+    C o[4] = {a[1], a[3], a[5], a[7]};
+    dft<4, DIR, T>(e);
+    dft<4, DIR, T>(o);
+    // o[k] *= W8^k, W8 = exp(DIR * i * pi / 4)
+    C o1 = cmul_i<DIR>(o[1]);          // o1 * (DIR i)
+    o1 = C{(o[1].x + o1.x) * h, (o[1].y + o1.y) * h};  // (1 + DIR i) h
+    C o2 = cmul_i<DIR>(o[2]);
+    C o3i = cmul_i<DIR>(o[3]);
+    C o3 = C{(o3i.x - o[3].x) * h, (o3i.y - o[3].y) * h};  // (-1 + DIR i) h
+    a[0] = cadd(e[0], o[0]);
+    a[4] = csub(e[0], o[0]);
+    a[1] = cadd(e[1], o1);
+    a[5] = csub(e[1], o1);
+    a[2] = cadd(e[2], o2);
+    a[6] = csub(e[2], o2);
+    a[3] = cadd(e[3], o3);
+    a[7] = csub(e[3], o3);
+  }
+}
+
+// One transform of length N held by TPT = N/P threads, P points each.
+// `lds` addresses position pos of this transform at lds[pos * lstride]
+// (the caller offsets the pointer per transform); `tw` is the plan's twiddle
+// table (sign -1 values; DIR > 0 uses their conjugates).  Every thread of the
+// workgroup must call this (it contains barriers).
+template <typename T, int N, int P, int DIR, int S = 0>
+__device__ __forceinline__ void fft_stages(c2_t<T> (&v)[P], c2_t<T>* lds, int lstride, int t,
+                                           const c2_t<T>* __restrict__ tw) {
+  using C = c2_t<T>;
+  constexpr PlanDesc pd = make_plan(N, P);
+  static_assert(pd.nst > 0, "unsupported FFT length");
+  if constexpr (S < pd.nst) {
+    constexpr int R = pd.r[S];
+    constexpr int L = pd.L[S];
+    constexpr int TPT = N / P;
+    constexpr int NB = P / R;
+    constexpr bool last = (S == pd.nst - 1);
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int j = t + q * TPT;
+      C a[R];
+#pragma unroll
+      for (int m = 0; m < R; ++m) a[m] = v[q + m * NB];
+      if constexpr (S > 0) {
+        const int k = j % L;
+        const C* w = tw + pd.tw_off[S] + k * (R - 1);
+#pragma unroll
+        for (int m = 1; m < R; ++m) {
+          C wm = w[m - 1];
+          if (DIR > 0) wm.y = -wm.y;
+          a[m] = cmul(a[m], wm);
+        }
+      }
+      dft<R, DIR, T>(a);
+      if constexpr (last) {
+#pragma unroll
+        for (int m = 0; m < R; ++m) v[q + m * NB] = a[m];
+      } else {
+        const int k = j % L;
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int m = 0; m < R; ++m) lds[(base + m * L) * lstride] = a[m];
+      }
+    }
+    if constexpr (!last) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < P; ++i) v[i] = lds[(t + i * TPT) * lstride];
+      __syncthreads();
+      fft_stages<T, N, P, DIR, S + 1>(v, lds, lstride, t, tw);
+    }
+  }
+}
+
+// Wavefront + workgroup sum of two doubles; result valid in thread 0.
+// `red` must hold 2 * (blockDim/64) doubles of LDS.
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[2 * w] = a;
+    red[2 * w + 1] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0, sb = 0;
+    for (int i = 0; i < nw; ++i) {
+      sa += red[2 * i];
+      sb += red[2 * i + 1];
+    }
+    a = sa;
+    b = sb;
+  }
+}
+
+__device__ __forceinline__ double block_max(double a, double* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(a, off, 64);
+    a = o > a ? o : a;
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+  __syncthreads();
+  if (lane == 0) red[w] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int i = 1; i < nw; ++i) m = red[i] > m ? red[i] : m;
+    a = m;
+  }
+  return a;
+}
+
+}  // namespace nadia

@@ -1,0 +1,51 @@
+// kern_common.hip -- single-workgroup final reductions (fixed order, so the
+// results are deterministic for a given launch geometry).
+#include "fft_device.h"
+#include "kernels.h"
+
+namespace nadia {
+
+__global__ void __launch_bounds__(256) final_sum2_kernel(const double* p, int n, double* ratio, double* pair) {
+  __shared__ double red[16];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    a += p[2 * i];
+    b += p[2 * i + 1];
+  }
+  block_sum2(a, b, red);
+  if (threadIdx.x == 0) {
+    if (ratio) ratio[0] = a / b;
+    if (pair) {
+      pair[0] = a;
+      pair[1] = b;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) final_max_kernel(const double* p, int n, double* out) {
+  __shared__ double red[8];
+  double m = -INFINITY;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) m = p[i] > m ? p[i] : m;
+  m = block_max(m, red);
+  if (threadIdx.x == 0) out[0] = m;
+}
+
+__global__ void __launch_bounds__(256) final_sum1_kernel(const double* p, int n, double* out) {
+  __shared__ double red[16];
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) a += p[i];
+  block_sum2(a, b, red);
+  if (threadIdx.x == 0) out[0] = a;
+}
+
+void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st) {
+  hipLaunchKernelGGL(final_sum2_kernel, dim3(1), dim3(256), 0, st, partials, n, out_ratio, out_pair);
+}
+void k_final_max(const double* partials, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(final_max_kernel, dim3(1), dim3(256), 0, st, partials, n, out);
+}
+void k_final_sum1(const double* partials, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(final_sum1_kernel, dim3(1), dim3(256), 0, st, partials, n, out);
+}
+
+}  // namespace nadia

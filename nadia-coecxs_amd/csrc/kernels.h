@@ -1,0 +1,111 @@
+// kernels.h -- argument blocks and launchers of the HIP kernels (host view).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace nadia {
+
+// Row pass: transforms along y (contiguous, length ny) of every row x.
+enum RowMode {
+  ROW_FFT = 0,     // dst = scale * DFT_dir(pre(src)) ; pre = optional (-1)^(x+y)
+  ROW_UPDATE = 1,  // inverse row DFT -> P_F x -> algorithm update of x -> forward row DFT
+  ROW_WF2 = 2,     // white field: inverse row DFT -> support -> forward row DFT
+  ROW_WF4 = 3,     // white field: forward row DFT -> modulus -> store x -> inverse row DFT
+};
+
+// Column pass: transforms along x (stride ny, length nx) of every column y.
+enum ColMode {
+  COL_FFT = 0,      // buf = scale * DFT_dir(buf)
+  COL_PLANAR = 1,   // forward column DFT -> far-field modulus projection -> inverse
+  COL_FRESNEL = 2,  // forward column DFT -> Fresnel phase, white field, modulus -> inverse
+  COL_WF1 = 3,      // white field: inverse column DFT -> x sf*coef -> inverse column DFT
+  COL_WF3 = 4,      // white field: forward column DFT -> x conj(coef) -> forward column DFT
+};
+
+struct RowArgs {
+  int mode;
+  int nrows, ny;
+  const void* tw;  // twiddle table of length ny (sign -1 values)
+  // ROW_FFT
+  const void* src;
+  void* dst;
+  int dir;
+  int pre_sign;  // multiply by (-1)^(x+y) before the transform
+  double scale;
+  // ROW_UPDATE (x, support; W2 carries the P_F P_S x field when b != 0)
+  void* x;
+  const void* s;
+  const void* src2;
+  void* dst2;
+  double a, b, c, d, e;  // PSF, PFS, PS, PF, PI coefficients
+  int er;
+  double post;  // factor applied after the inverse row transform
+  // error finalisation of the preceding column pass (block 0)
+  const double* fin_partials;
+  int fin_n;
+  double* err_out;
+  // ROW_WF4: modulus inputs and partial sums of this pass
+  const void* q;
+  double* partials;
+};
+
+struct ColArgs {
+  int mode;
+  int nx, ny;
+  const void* tw;  // twiddle table of length nx
+  void* buf;
+  long long field_stride;  // elements between fields (blockIdx.y selects the field)
+  int dir;
+  double scale;
+  const void* q;    // sqrt(I) with the beam-stop mask in the sign bit
+  const void* ill;  // Fresnel white field in the detector plane
+  double sf;        // 1/sqrt(nx*ny)
+  double phase_factor, xmid, ymid;      // Fresnel quadratic phase
+  double wf_sx, wf_sy, wf_z12, wf_z23, wf_k, wf_norm;  // white-field phase and 1/sqrt(N)
+  double* partials;  // 2 doubles per block per field
+};
+
+struct ConvArgs {
+  const void* src;  // complex (magnitude taken) or real
+  int src_complex;
+  void* dst;        // real
+  int nx, ny;
+  const void* g;    // (cut+1)^2 window in the field's real type
+  double* partials; // one max per block
+};
+
+template <typename T> void launch_row(const RowArgs& a, hipStream_t st);
+template <typename T> void launch_col(const ColArgs& a, int nfields, hipStream_t st);
+template <typename T> int col_tile(int nx);
+template <typename T> int row_tile(int ny);
+template <typename T> int row_blocks(int nx, int ny);
+
+// element-wise and reduction kernels (kern_misc.inc)
+template <typename T> void k_invert(void* buf, int nx, int ny, int scale, hipStream_t st);
+template <typename T> void k_scale_intensity(void* buf, const void* q, int nx, int ny, double* partials, int nblocks, hipStream_t st);
+template <typename T> void k_fresnel_add(void* buf, const void* ill, double s, long long n, hipStream_t st);
+template <typename T> void k_support(void* buf, const void* s, long long n, hipStream_t st);
+template <typename T> void k_combine(void* x, const void* pfs, const void* pf, const void* ps, const void* psf,
+                                     const double* coef, long long n, hipStream_t st);
+template <typename T> void k_mulfac_fresnel(void* buf, int nx, int ny, double factor, int dir, hipStream_t st);
+template <typename T> void k_mulfac_wf(void* buf, int nx, int ny, double sx, double sy, double z12, double z23,
+                                       double k, double nrm, int dir, hipStream_t st);
+template <typename T> void k_scale(void* buf, double s, long long n, hipStream_t st);
+template <typename T> void k_norm2(const void* buf, long long n, double* partials, int nblocks, hipStream_t st);
+template <typename T> void k_magnitude(const void* buf, void* out, long long n, hipStream_t st);
+template <typename T> void k_make_q(const void* intensity, const void* beam, int has_beam, void* q, long long n, hipStream_t st);
+template <typename T> void k_sqrt_to_real(const void* q, void* out, long long n, hipStream_t st);
+template <typename T> void k_autocorr_prep(const void* q, void* out, long long n, hipStream_t st);
+template <typename T> void k_esw_from_trans(const void* trans, const void* ill, void* esw, int nx, int ny, hipStream_t st);
+template <typename T> void k_trans_from_esw(const void* esw, const void* ill, void* out, long long n, hipStream_t st);
+template <typename T> void k_conv(const ConvArgs& a, int cut, int* nblocks, hipStream_t st);
+template <typename T> void k_real_max(const void* src, long long n, double* partials, int nblocks, hipStream_t st);
+template <typename T> void k_threshold(const void* conv, void* out, int nx, int ny, const double* maxv, double thr,
+                                       hipStream_t st);
+template <typename T> void k_normalise(const void* src, void* dst, const double* maxv, long long n, hipStream_t st);
+template <typename T> void k_wf_disk(void* s, int nx, int ny, double centre, double dz, double radius, hipStream_t st);
+void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st);
+void k_final_max(const double* partials, int n, double* out, hipStream_t st);
+void k_final_sum1(const double* partials, int n, double* out, hipStream_t st);
+
+constexpr int kEwBlocks = 1024;  // grid size of grid-stride reductions
+}  // namespace nadia

@@ -1,0 +1,87 @@
+// plan.h -- compile-time Stockham FFT plans, shared by host and device code.
+//
+// A transform of length N is computed by TPT = N/P threads, each holding P
+// points in registers.  Register slot i of thread t always holds position
+// t + i*TPT at the start and at the end of a transform, so the first stage
+// reads and the last stage writes straight from/to global memory (coalesced),
+// and only the intermediate stages exchange data through LDS.
+//
+// Stage s has radix r[s] and span L[s] (product of the earlier radices).
+// Butterfly j (0 <= j < N/r) reads positions j + m*N/r, multiplies input m by
+// W_{L r}^{(j mod L) m}, runs a size-r DFT and writes position
+// (j div L)*L*r + (j mod L) + m*L (the Stockham autosort ordering).
+#pragma once
+
+namespace nadia {
+
+constexpr int kMaxStages = 12;
+
+struct PlanDesc {
+  int n, p, tpt, nst;
+  int r[kMaxStages];
+  int L[kMaxStages];
+  int tw_off[kMaxStages + 1];  // offsets (in complex elements) of each stage's twiddles
+};
+
+// Radices are taken from {8, 4, 3, 2}; each must divide both the remaining
+// length and P (every thread runs P/r butterflies of radix r per stage).
+constexpr PlanDesc make_plan(int n, int p) {
+  PlanDesc d{};
+  d.n = n;
+  d.p = p;
+  d.tpt = n / p;
+  int m = n, L = 1, s = 0, off = 0;
+  while (m > 1 && s < kMaxStages) {
+    int r = 0;
+    const int cand[4] = {8, 4, 3, 2};
+    for (int c = 0; c < 4; ++c)
+      if (r == 0 && m % cand[c] == 0 && p % cand[c] == 0) r = cand[c];
+    if (r == 0) break;  // unsupported length: caught by plan_ok
+    d.r[s] = r;
+    d.L[s] = L;
+    d.tw_off[s] = off;
+    off += L * (r - 1);
+    L *= r;
+    m /= r;
+    ++s;
+  }
+  d.nst = (m == 1) ? s : -1;
+  d.tw_off[s < kMaxStages ? s : kMaxStages] = off;
+  return d;
+}
+
+constexpr int plan_twiddles(const PlanDesc& d) { return d.nst > 0 ? d.tw_off[d.nst] : 0; }
+
+// Lengths with compiled kernels: 2^k (8..4096) and 3*2^k (24..3072).
+constexpr bool size_supported(int n) {
+  return n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 ||
+         n == 1024 || n == 2048 || n == 4096 || n == 24 || n == 48 || n == 96 ||
+         n == 192 || n == 384 || n == 768 || n == 1536 || n == 3072;
+}
+
+// Points per thread: row passes (contiguous transforms).
+constexpr int row_points(int n) { return (n % 3 == 0) ? (n < 24 ? n : 24) : (n < 8 ? n : 8); }
+
+// Points per thread for column passes (same radix plans as the rows; more
+// points per thread spills registers once the tile has 1024 threads).
+constexpr int col_points(int n) { return row_points(n); }
+
+// Transforms (rows) per row-pass workgroup: about 256 threads.
+constexpr int row_batch(int n) {
+  int tpt = n / row_points(n);
+  int b = 256 / tpt;
+  return b < 1 ? 1 : b;
+}
+
+// Columns per column-pass workgroup for element size `esz` (8 = fp32 complex,
+// 16 = fp64 complex): up to 64-byte row segments, LDS tile N*C*esz <= 128 KiB,
+// at most 1024 threads (512 in fp64, whose projection epilogues need more
+// than the 128 registers a 1024-thread workgroup leaves each thread).
+constexpr int col_batch(int n, int esz) {
+  int c = 64 / esz;
+  const int maxthr = esz > 8 ? 512 : 1024;
+  while (c > 1 && (n * c * esz > 128 * 1024 || c * (n / col_points(n)) > maxthr)) c /= 2;
+  return c;
+}
+
+}  // namespace nadia

@@ -1,0 +1,821 @@
+/*
+ * nadia_oracle.c -- CPU restatement of NADIA's iterative phase-retrieval path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP library: it
+ * is imported by tests/, by __graft_entry__.smoke() and by bench.py's
+ * cpu_baseline leg, never by the product path.
+ *
+ * Every function follows one reference function; the citation is given as
+ * NADIA/<file>:<line> (NADIA v0.5).  The arithmetic type `real` is double by
+ * default (the reference built with -DDOUBLE_PRECISION, i.e. libNADIAd) and
+ * float with -DORACLE_FP32 (the default libNADIA build: float arrays, double
+ * temporaries and double error accumulators).
+ *
+ * The reference links FFTW3, which this image does not have; the 2-D DFT here
+ * is this file's own mixed-radix Stockham FFT implementing FFTW's documented
+ * definition (unnormalised, exponent sign -1 for FFTW_FORWARD, row-major nx x ny).
+ *
+ * Pinning (tests/test_oracle_*.py): brute-force DFT, numpy.fft, closed forms
+ * (delta <-> constant, plane wave <-> delta, Parseval), fftshift identity,
+ * unitarity / round trips of every propagator, the named-algorithm coefficient
+ * table, fixed points of every algorithm, HIO's inside/outside rule, scipy's
+ * 2-D convolution for the shrinkwrap window, and the closed-form Fresnel and
+ * white-field phases.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+#ifdef ORACLE_FP32
+typedef float real;
+#else
+typedef double real;
+#endif
+
+typedef struct { real re, im; } cplx;
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* 1-D mixed-radix Stockham FFT (radix 4, 2, 3, 5 and generic odd factors).  */
+/* ------------------------------------------------------------------------ */
+
+#define MAXSTAGES 32
+
+typedef struct {
+  int n;
+  int nst;
+  int radix[MAXSTAGES];
+  int span[MAXSTAGES];      /* L: product of the radices of earlier stages */
+  double *tw[MAXSTAGES];    /* per stage: L*(r-1) complex twiddles, sign -1 */
+} fft_plan1;
+
+static void plan1_make(fft_plan1 *p, int n) {
+  int m = n, L = 1, s = 0;
+  p->n = n;
+  while (m > 1) {
+    int r;
+    if (m % 4 == 0) r = 4;
+    else if (m % 2 == 0) r = 2;
+    else if (m % 3 == 0) r = 3;
+    else if (m % 5 == 0) r = 5;
+    else { r = 7; while (m % r) r += 2; }
+    p->radix[s] = r;
+    p->span[s] = L;
+    p->tw[s] = (double *)malloc(sizeof(double) * 2 * (size_t)L * (r - 1) + 16);
+    for (int k = 0; k < L; k++)
+      for (int q = 1; q < r; q++) {
+        /* exp(-2 pi i k q / (L r)), argument reduced exactly in integers */
+        long t = ((long)k * q) % ((long)L * r);
+        double a = -2.0 * M_PI * (double)t / (double)(L * r);
+        p->tw[s][2 * (k * (r - 1) + q - 1)] = cos(a);
+        p->tw[s][2 * (k * (r - 1) + q - 1) + 1] = sin(a);
+      }
+    L *= r;
+    m /= r;
+    s++;
+  }
+  p->nst = s;
+}
+
+static void plan1_free(fft_plan1 *p) {
+  for (int s = 0; s < p->nst; s++) free(p->tw[s]);
+  p->nst = 0;
+}
+
+/* Small DFT of size r on v (double), sign = -1 or +1. */
+static void small_dft(double *vr, double *vi, int r, int sign) {
+  if (r == 2) {
+    double ar = vr[0], ai = vi[0];
+    vr[0] = ar + vr[1]; vi[0] = ai + vi[1];
+    vr[1] = ar - vr[1]; vi[1] = ai - vi[1];
+  } else if (r == 4) {
+    double t0r = vr[0] + vr[2], t0i = vi[0] + vi[2];
+    double t1r = vr[0] - vr[2], t1i = vi[0] - vi[2];
+    double t2r = vr[1] + vr[3], t2i = vi[1] + vi[3];
+    double t3r = vr[1] - vr[3], t3i = vi[1] - vi[3];
+    /* multiply t3 by sign*i */
+    double ur = -sign * t3i, ui = sign * t3r;
+    vr[0] = t0r + t2r; vi[0] = t0i + t2i;
+    vr[2] = t0r - t2r; vi[2] = t0i - t2i;
+    vr[1] = t1r + ur;  vi[1] = t1i + ui;
+    vr[3] = t1r - ur;  vi[3] = t1i - ui;
+  } else {
+    double bufr[64], bufi[64];
+    if (r > 64) { fprintf(stderr, "oracle: radix too large\n"); exit(1); }
+    for (int k = 0; k < r; k++) {
+      double sr = 0, si = 0;
+      for (int j = 0; j < r; j++) {
+        long t = ((long)j * k) % r;
+        double a = sign * 2.0 * M_PI * (double)t / (double)r;
+        double c = cos(a), s = sin(a);
+        sr += vr[j] * c - vi[j] * s;
+        si += vr[j] * s + vi[j] * c;
+      }
+      bufr[k] = sr; bufi[k] = si;
+    }
+    for (int k = 0; k < r; k++) { vr[k] = bufr[k]; vi[k] = bufi[k]; }
+  }
+}
+
+/* In-place 1-D DFT of x (n points, stride 1) using scratch y (n points). */
+static void fft1(const fft_plan1 *p, cplx *x, cplx *y, int sign) {
+  int n = p->n;
+  cplx *in = x, *out = y;
+  double vr[64], vi[64];
+  for (int s = 0; s < p->nst; s++) {
+    int r = p->radix[s], L = p->span[s], nr = n / r;
+    const double *tw = p->tw[s];
+    for (int j = 0; j < nr; j++) {
+      int k = j % L;
+      for (int q = 0; q < r; q++) {
+        double ar = in[j + q * nr].re, ai = in[j + q * nr].im;
+        if (q > 0 && k > 0) {
+          /* tw holds the sign -1 twiddle; sign +1 uses its conjugate */
+          double c = tw[2 * (k * (r - 1) + q - 1)];
+          double d = (sign < 0) ? tw[2 * (k * (r - 1) + q - 1) + 1] : -tw[2 * (k * (r - 1) + q - 1) + 1];
+          double tr = ar * c - ai * d, ti = ar * d + ai * c;
+          ar = tr; ai = ti;
+        }
+        vr[q] = ar; vi[q] = ai;
+      }
+      small_dft(vr, vi, r, sign);
+      int base = (j / L) * L * r + k;
+      for (int q = 0; q < r; q++) {
+        out[base + q * L].re = (real)vr[q];
+        out[base + q * L].im = (real)vi[q];
+      }
+    }
+    cplx *t = in; in = out; out = t;
+  }
+  if (in != x) memcpy(x, in, sizeof(cplx) * (size_t)n);
+}
+
+/* Unnormalised in-place 2-D DFT of the nx x ny x-major array a (element
+ * (i,j) at a[i*ny+j]); sign -1 = FFTW_FORWARD, +1 = FFTW_BACKWARD.
+ * Replaces FFTW's fftw_plan_dft_2d/fftw_execute at NADIA/src/Complex_2D.c++:462-503. */
+void or_fft2d(cplx *a, int nx, int ny, int sign) {
+  fft_plan1 px, py;
+  plan1_make(&py, ny);
+  plan1_make(&px, nx);
+  int nmax = nx > ny ? nx : ny;
+  const int B = 8; /* columns gathered per block */
+  cplx *buf = (cplx *)malloc(sizeof(cplx) * (size_t)nmax * B);
+  cplx *scr = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+  for (int i = 0; i < nx; i++) fft1(&py, a + (size_t)i * ny, scr, sign);
+  for (int j0 = 0; j0 < ny; j0 += B) {
+    int nb = (ny - j0) < B ? (ny - j0) : B;
+    for (int i = 0; i < nx; i++)
+      for (int b = 0; b < nb; b++) buf[(size_t)b * nx + i] = a[(size_t)i * ny + j0 + b];
+    for (int b = 0; b < nb; b++) fft1(&px, buf + (size_t)b * nx, scr, sign);
+    for (int i = 0; i < nx; i++)
+      for (int b = 0; b < nb; b++) a[(size_t)i * ny + j0 + b] = buf[(size_t)b * nx + i];
+  }
+  free(buf); free(scr);
+  plan1_free(&px); plan1_free(&py);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Complex_2D / Double_2D operations                                         */
+/* ------------------------------------------------------------------------ */
+
+/* fftshift with 1/sqrt(nx*ny) scaling: NADIA/src/Complex_2D.c++:317-353 */
+void or_invert(cplx *a, int nx, int ny, int scale) {
+  int mx = nx / 2, my = ny / 2;
+  real sf = 1;
+  if (scale) sf = (real)(1.0 / sqrt((double)(nx * ny)));
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < my; ++j) {
+      int jn = j + my, in = i + mx;
+      if (i >= mx) in -= 2 * mx;
+      cplx t = a[(size_t)in * ny + jn];
+      a[(size_t)in * ny + jn].re = a[(size_t)i * ny + j].re * sf;
+      a[(size_t)in * ny + jn].im = a[(size_t)i * ny + j].im * sf;
+      a[(size_t)i * ny + j].re = t.re * sf;
+      a[(size_t)i * ny + j].im = t.im * sf;
+    }
+}
+
+/* |c|: NADIA/include/Complex_2D.h:265-268 (computed in real) */
+static inline real c_mag(cplx c) { return (real)sqrt(c.re * c.re + c.im * c.im); }
+
+/* set_mag: NADIA/include/Complex_2D.h:205-215 */
+static inline void c_set_mag(cplx *c, real value) {
+  double mag = c_mag(*c);
+  if (mag == 0) { c->re = value; c->im = 0; }
+  else { c->re = (real)(c->re * (value / mag)); c->im = (real)(c->im * (value / mag)); }
+}
+
+/* Complex_2D::add: NADIA/src/Complex_2D.c++:183-199 */
+void or_add(cplx *a, const cplx *b, size_t n, real scale) {
+  for (size_t p = 0; p < n; p++) { a[p].re += scale * b[p].re; a[p].im += scale * b[p].im; }
+}
+
+/* Complex_2D::scale: NADIA/src/Complex_2D.c++:169-180 */
+void or_scale(cplx *a, size_t n, real s) {
+  for (size_t p = 0; p < n; p++) { a[p].re *= s; a[p].im *= s; }
+}
+
+/* Complex_2D::get_norm: NADIA/src/Complex_2D.c++:259-270 */
+real or_get_norm(const cplx *a, size_t n) {
+  real s = 0;
+  for (size_t p = 0; p < n; p++) s += (real)(pow(a[p].im, 2) + pow(a[p].re, 2));
+  return (real)sqrt(s);
+}
+
+/* Real_2D::get_max: NADIA/include/Double_2D.h:252-265 */
+static real r_max(const real *a, size_t n) {
+  if (n == 0) return 0;
+  real m = a[0];
+  for (size_t p = 0; p < n; p++) if (a[p] > m) m = a[p];
+  return m;
+}
+
+/* Real_2D::get_sum: NADIA/include/Double_2D.h:210-216 */
+static real r_sum(const real *a, size_t n) {
+  real t = 0;
+  for (size_t p = 0; p < n; p++) t += a[p];
+  return t;
+}
+
+/* ------------------------------------------------------------------------ */
+/* BaseCDI state                                                             */
+/* ------------------------------------------------------------------------ */
+
+enum { PSF, PFS, PS, PF, PI_ };
+enum { ER, BIO, BOO, HIO, DM, SF, ASR, HPR, RAAR, CUSTOM };
+enum { KIND_PLANAR = 0, KIND_FRESNEL = 1, KIND_WF = 2 };
+
+typedef struct {
+  int kind, nx, ny;
+  cplx *complex;            /* the caller's estimate, mutated in place */
+  real *support, *intensity_sqrt, *beam_stop;
+  double beta;
+  int algorithm;
+  double alg[5];
+  double current_error;
+  cplx *tPFS, *tPF, *tPS, *tPSF;
+  int n_best;
+  cplx **best;
+  double *best_err;
+  /* FresnelCDI */
+  cplx *illumination, *illum_at_sample, *coefficient, *transmission;
+  double norm, wavelength, pixel, zfd, zfs;
+  /* FresnelCDI_WF */
+  double z12, z23;
+} ocdi;
+
+#define NPIX(o) ((size_t)(o)->nx * (o)->ny)
+
+static cplx *calloc_c(size_t n) { return (cplx *)calloc(n ? n : 1, sizeof(cplx)); }
+static real *calloc_r(size_t n) { return (real *)calloc(n ? n : 1, sizeof(real)); }
+
+void ocdi_set_algorithm(ocdi *o, int alg);
+
+/* BaseCDI::BaseCDI: NADIA/src/BaseCDI.c++:33-66 */
+ocdi *ocdi_create(cplx *initial_guess, int nx, int ny, int kind, int n_best) {
+  ocdi *o = (ocdi *)calloc(1, sizeof(ocdi));
+  o->kind = kind; o->nx = nx; o->ny = ny;
+  o->complex = initial_guess;
+  o->beta = 0.9;
+  o->support = calloc_r(NPIX(o));
+  o->intensity_sqrt = calloc_r(NPIX(o));
+  o->n_best = n_best;
+  if (n_best > 0) {
+    o->best = (cplx **)malloc(sizeof(cplx *) * n_best);
+    o->best_err = (double *)malloc(sizeof(double) * n_best);
+    for (int i = 0; i < n_best; i++) { o->best[i] = calloc_c(NPIX(o)); o->best_err[i] = 1; }
+  }
+  o->algorithm = CUSTOM;
+  ocdi_set_algorithm(o, ER);
+  return o;
+}
+
+void ocdi_destroy(ocdi *o) {
+  if (!o) return;
+  free(o->support); free(o->intensity_sqrt); free(o->beam_stop);
+  free(o->tPFS); free(o->tPF); free(o->tPS); free(o->tPSF);
+  for (int i = 0; i < o->n_best; i++) free(o->best[i]);
+  free(o->best); free(o->best_err);
+  free(o->illumination); free(o->illum_at_sample); free(o->coefficient); free(o->transmission);
+  free(o);
+}
+
+/* BaseCDI::convolve: NADIA/src/BaseCDI.c++:432-474 (unnormalised truncated
+ * Gaussian, zero outside the array, double accumulator, window stored in real) */
+void ocdi_convolve(ocdi *o, real *array, double gauss_width, int cut) {
+  int nx = o->nx, ny = o->ny;
+  real *tmp = calloc_r(NPIX(o));
+  real *g = calloc_r((size_t)(cut + 1) * (cut + 1));
+  for (int i = 0; i <= cut; i++)
+    for (int j = 0; j <= cut; j++) {
+      double denom = 2.0 * gauss_width * gauss_width;
+      g[i * (cut + 1) + j] = (real)exp(-1 * (i * i + j * j) / denom);
+    }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      double v = 0;
+      for (int i2 = i - cut; i2 <= i + cut; i2++)
+        for (int j2 = j - cut; j2 <= j + cut; j2++)
+          if (i2 < nx && i2 >= 0 && j2 >= 0 && j2 < ny)
+            v += (double)array[(size_t)i2 * ny + j2] * (double)g[abs(i - i2) * (cut + 1) + abs(j - j2)];
+      tmp[(size_t)i * ny + j] = (real)v;
+    }
+  memcpy(array, tmp, sizeof(real) * NPIX(o));
+  free(tmp); free(g);
+}
+
+/* BaseCDI::set_support: NADIA/src/BaseCDI.c++:95-108 */
+void ocdi_set_support(ocdi *o, const real *s, int soften) {
+  double max = r_max(s, NPIX(o));
+  for (size_t p = 0; p < NPIX(o); p++) o->support[p] = (real)(s[p] / max);
+  if (soften) ocdi_convolve(o, o->support, 3, 5);
+}
+
+/* BaseCDI::set_beam_stop: NADIA/src/BaseCDI.c++:112-116 */
+void ocdi_set_beam_stop(ocdi *o, const real *b) {
+  if (!o->beam_stop) o->beam_stop = calloc_r(NPIX(o));
+  memcpy(o->beam_stop, b, sizeof(real) * NPIX(o));
+}
+
+/* BaseCDI::set_intensity: NADIA/src/BaseCDI.c++:118-124 */
+void ocdi_set_intensity(ocdi *o, const real *I) {
+  for (size_t p = 0; p < NPIX(o); p++) o->intensity_sqrt[p] = (real)sqrt(I[p]);
+}
+
+/* BaseCDI::get_error: NADIA/src/BaseCDI.c++:126-128 */
+double ocdi_get_error(ocdi *o) { return o->current_error; }
+
+/* BaseCDI::support_constraint: NADIA/src/BaseCDI.c++:136-154 */
+static void support_constraint(ocdi *o, cplx *c) {
+  for (size_t p = 0; p < NPIX(o); p++) {
+    double s = o->support[p];
+    if (s == 0) { c[p].re = 0; c[p].im = 0; }
+    else if (s < 1) { c[p].re = (real)(c[p].re * s); c[p].im = (real)(c[p].im * s); }
+  }
+}
+
+/* ---- FresnelCDI transmission helpers (no TransmissionConstraint) ------- */
+
+/* FresnelCDI::set_transmission_function: NADIA/src/FresnelCDI.c++:253-283
+ * (the loop runs j < nx: square arrays only, as in the reference) */
+void ocdi_set_transmission_function(ocdi *o, const cplx *T, cplx *esw) {
+  int nx = o->nx, ny = o->ny;
+  if (!esw) esw = o->complex;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < nx; j++) {
+      size_t p = (size_t)i * ny + j;
+      double lr = o->illum_at_sample[p].re, li = o->illum_at_sample[p].im;
+      double tr = T[p].re, ti = T[p].im;
+      esw[p].re = (real)(lr * tr - li * ti - lr);
+      esw[p].im = (real)(lr * ti + li * tr - li);
+    }
+}
+
+/* FresnelCDI::get_transmission_function: NADIA/src/FresnelCDI.c++:298-355 */
+void ocdi_get_transmission_function(ocdi *o, cplx *result, const cplx *esw) {
+  if (!esw) esw = o->complex;
+  for (size_t p = 0; p < NPIX(o); p++) {
+    double lr = o->illum_at_sample[p].re, li = o->illum_at_sample[p].im;
+    double den = lr * lr + li * li;
+    if (den != 0) {
+      double er = esw[p].re, ei = esw[p].im;
+      double rn = lr * (er + lr) + li * (ei + li);
+      double in = lr * (ei + li) - li * (er + lr);
+      result[p].re = (real)(rn / den);
+      result[p].im = (real)(in / den);
+    } else { result[p].re = 1.0; result[p].im = 0.0; }
+  }
+}
+
+/* ---- virtual methods dispatched on kind ----------------------------- */
+
+/* FresnelCDI::multiply_factors: NADIA/src/FresnelCDI.c++:98-126, and
+ * FresnelCDI_WF::multiply_factors: NADIA/src/FresnelCDI_WF.c++:70-96
+ * (coefficient table nx/2 x ny/2 mirrored about (n-1)/2) */
+static void multiply_factors(ocdi *o, cplx *c, int direction) {
+  int nx = o->nx, ny = o->ny;
+  double xm = (nx - 1) / 2.0, ym = (ny - 1) / 2.0;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      size_t p = (size_t)i * ny + j;
+      double orr = c[p].re, oi = c[p].im;
+      int i_ = (int)(xm - fabs(xm - i));
+      int j_ = (int)(ym - fabs(ym - j));
+      double cr = o->coefficient[(size_t)i_ * (ny / 2) + j_].re;
+      double ci = direction * (double)o->coefficient[(size_t)i_ * (ny / 2) + j_].im;
+      c[p].re = (real)(orr * cr - oi * ci);
+      c[p].im = (real)(oi * cr + orr * ci);
+    }
+}
+
+#define FORWARD (-1)
+#define BACKWARD (+1)
+
+/* PlanarCDI::propagate_to_detector: NADIA/src/PlanarCDI.c++:65-68
+ * FresnelCDI::propagate_to_detector: NADIA/src/FresnelCDI.c++:245-250
+ * FresnelCDI_WF::propagate_to_detector: NADIA/src/FresnelCDI_WF.c++:164-175 */
+void ocdi_propagate_to_detector(ocdi *o, cplx *c) {
+  int nx = o->nx, ny = o->ny;
+  if (o->kind == KIND_PLANAR) {
+    or_fft2d(c, nx, ny, -1);
+    or_invert(c, nx, ny, 1);
+  } else if (o->kind == KIND_FRESNEL) {
+    or_invert(c, nx, ny, 1);
+    or_fft2d(c, nx, ny, -1);
+    multiply_factors(o, c, FORWARD);
+  } else {
+    or_fft2d(c, nx, ny, -1);
+    multiply_factors(o, c, FORWARD);
+    or_invert(c, nx, ny, 1);
+    or_fft2d(c, nx, ny, -1);
+  }
+}
+
+/* PlanarCDI::propagate_from_detector: NADIA/src/PlanarCDI.c++:70-73
+ * FresnelCDI::propagate_from_detector: NADIA/src/FresnelCDI.c++:238-243
+ * FresnelCDI_WF::propagate_from_detector: NADIA/src/FresnelCDI_WF.c++:152-162 */
+void ocdi_propagate_from_detector(ocdi *o, cplx *c) {
+  int nx = o->nx, ny = o->ny;
+  if (o->kind == KIND_PLANAR) {
+    or_invert(c, nx, ny, 1);
+    or_fft2d(c, nx, ny, +1);
+  } else if (o->kind == KIND_FRESNEL) {
+    multiply_factors(o, c, BACKWARD);
+    or_fft2d(c, nx, ny, +1);
+    or_invert(c, nx, ny, 1);
+  } else {
+    or_fft2d(c, nx, ny, +1);
+    or_invert(c, nx, ny, 1);
+    multiply_factors(o, c, BACKWARD);
+    or_fft2d(c, nx, ny, +1);
+  }
+}
+
+/* BaseCDI::scale_intensity: NADIA/src/BaseCDI.c++:163-189 */
+static void base_scale_intensity(ocdi *o, cplx *c) {
+  double norm2_mag = 0, norm2_diff = 0;
+  for (size_t p = 0; p < NPIX(o); p++) {
+    if (o->beam_stop == NULL || o->beam_stop[p] > 0) {
+      double is = o->intensity_sqrt[p];
+      double mag = c_mag(c[p]);
+      c_set_mag(&c[p], (real)is);
+      norm2_mag += is * is;
+      norm2_diff += (mag - is) * (mag - is);
+    }
+  }
+  o->current_error = norm2_diff / norm2_mag;
+}
+
+/* BaseCDI::scale_intensity, FresnelCDI::scale_intensity
+ * (NADIA/src/FresnelCDI.c++:217-236: add white field, project, subtract) */
+void ocdi_scale_intensity(ocdi *o, cplx *c) {
+  if (o->kind == KIND_FRESNEL) {
+    or_add(c, o->illumination, NPIX(o), 1);
+    base_scale_intensity(o, c);
+    or_add(c, o->illumination, NPIX(o), -1.0);
+  } else {
+    base_scale_intensity(o, c);
+  }
+}
+
+/* BaseCDI::apply_support: NADIA/src/BaseCDI.c++:130-134;
+ * FresnelCDI::apply_support: NADIA/src/FresnelCDI.c++:203-215 (the
+ * transmission round trip only runs with a TransmissionConstraint, which this
+ * oracle does not model) */
+void ocdi_apply_support(ocdi *o, cplx *c) { support_constraint(o, c); }
+
+/* BaseCDI::project_intensity: NADIA/src/BaseCDI.c++:156-160 */
+void ocdi_project_intensity(ocdi *o, cplx *c) {
+  ocdi_propagate_to_detector(o, c);
+  ocdi_scale_intensity(o, c);
+  ocdi_propagate_from_detector(o, c);
+}
+
+/* BaseCDI::set_custom_algorithm: NADIA/src/BaseCDI.c++:240-255 plus
+ * reallocate_temp_complex_memory :257-275 */
+void ocdi_set_custom_algorithm(ocdi *o, double m1, double m2, double m3, double m4, double m5,
+                               double m6, double m7, double m8, double m9, double m10) {
+  o->alg[PSF] = m1 + m2 + m3 + m4;
+  o->alg[PFS] = -m1 + m5 + m6 + m7;
+  o->alg[PS] = -m3 - m6 - m8 + m10;
+  o->alg[PF] = -m2 - m5 + m8 + m9;
+  o->alg[PI_] = -m4 - m7 - m9 - m10;
+  cplx **t[4] = {&o->tPSF, &o->tPFS, &o->tPS, &o->tPF};
+  for (int n = 0; n < 4; n++) {
+    if (o->alg[n] == 0 && *t[n]) { free(*t[n]); *t[n] = NULL; }
+    if (o->alg[n] != 0 && !*t[n]) *t[n] = calloc_c(NPIX(o));
+  }
+  o->algorithm = CUSTOM;
+}
+
+/* BaseCDI::set_algorithm: NADIA/src/BaseCDI.c++:192-237 (early return when
+ * the algorithm is unchanged, so beta only applies on a change) */
+void ocdi_set_algorithm(ocdi *o, int alg) {
+  if (o->kind == KIND_WF && o->algorithm == ER) return; /* FresnelCDI_WF.h:147-150 hides it */
+  if (o->algorithm == alg) return;
+  double b = o->beta;
+  switch (alg) {
+    case ER: ocdi_set_custom_algorithm(o, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0); break;
+    case BIO: ocdi_set_custom_algorithm(o, 0, b, 0, 0, 0, 0, 0, 0, 0, 0); break;
+    case BOO: ocdi_set_custom_algorithm(o, 0, b, 0, 0, 0, 0, 0, 0, 1, 0); break;
+    case HIO: ocdi_set_custom_algorithm(o, 0, b, 1, 0, 0, 0, 0, 0, 0, 0); break;
+    case DM: ocdi_set_custom_algorithm(o, b, 0, 1, 0, 1, 0, 0, 0, 0, 0); break;
+    case SF: ocdi_set_custom_algorithm(o, 0, 1, 0, 1, 0, 0, 0, 0, 0, 0); break;
+    case ASR: ocdi_set_custom_algorithm(o, 0, 1, 1, 0, 0, 0, 0, 0, 0, 0); break;
+    case HPR: ocdi_set_custom_algorithm(o, 0, b, 1, 0, 0, 0, 0, 0, 0, 0); break;
+    case RAAR: ocdi_set_custom_algorithm(o, 0, 1, b, 0, 0, 0, 0, 0, (1 - b), 0); break;
+    default: fprintf(stdout, "Algorithm unknown\n");
+  }
+  o->algorithm = alg;
+}
+
+/* BaseCDI::set_relaxation_parameter: NADIA/include/BaseCDI.h:264-267 */
+void ocdi_set_relaxation_parameter(ocdi *o, double beta) {
+  if (o->kind == KIND_WF) return; /* FresnelCDI_WF.h:168-172 */
+  o->beta = beta;
+  ocdi_set_algorithm(o, o->algorithm);
+}
+
+void ocdi_get_coefficients(ocdi *o, double *out5, int *algorithm) {
+  for (int i = 0; i < 5; i++) out5[i] = o->alg[i];
+  if (algorithm) *algorithm = o->algorithm;
+}
+
+/* BaseCDI::update_n_best: NADIA/src/BaseCDI.c++:390-413 */
+static void update_n_best(ocdi *o) {
+  int place = 0;
+  for (; place < o->n_best && o->current_error > o->best_err[place]; place++);
+  if (o->n_best > 0 && place < o->n_best) {
+    cplx *tp = o->best[o->n_best - 1];
+    memcpy(tp, o->complex, sizeof(cplx) * NPIX(o));
+    for (int i = o->n_best - 1; i > place; i--) {
+      o->best_err[i] = o->best_err[i - 1];
+      o->best[i] = o->best[i - 1];
+    }
+    o->best_err[place] = o->current_error;
+    o->best[place] = tp;
+  }
+}
+
+/* BaseCDI::get_best_result: NADIA/src/BaseCDI.c++:82-91 */
+int ocdi_get_best_result(ocdi *o, int index, double *error, cplx *out) {
+  if (index >= 0 && index < o->n_best) {
+    *error = o->best_err[index];
+    memcpy(out, o->best[index], sizeof(cplx) * NPIX(o));
+    return 1;
+  }
+  return 0;
+}
+
+/* BaseCDI::reset_best: NADIA/include/BaseCDI.h:476-479 */
+void ocdi_reset_best(ocdi *o) { for (int i = 0; i < o->n_best; i++) o->best_err[i] = 1; }
+
+/* FresnelCDI_WF::iterate: NADIA/src/FresnelCDI_WF.c++:122-150 */
+static int wf_iterate(ocdi *o) {
+  ocdi_propagate_from_detector(o, o->complex);
+  ocdi_apply_support(o, o->complex);
+  ocdi_propagate_to_detector(o, o->complex);
+  ocdi_scale_intensity(o, o->complex);
+  return 1;
+}
+
+/* BaseCDI::iterate: NADIA/src/BaseCDI.c++:297-387 */
+int ocdi_iterate(ocdi *o) {
+  if (o->kind == KIND_WF) return wf_iterate(o);
+  cplx *x = o->complex;
+  size_t n = NPIX(o);
+  if (o->algorithm == ER) {
+    ocdi_project_intensity(o, x);
+    ocdi_apply_support(o, x);
+    update_n_best(o);
+    return 1;
+  }
+  if (o->alg[PFS] != 0) {
+    memcpy(o->tPFS, x, sizeof(cplx) * n);
+    ocdi_apply_support(o, o->tPFS);
+    ocdi_project_intensity(o, o->tPFS);
+  }
+  if (o->alg[PF] != 0) {
+    memcpy(o->tPF, x, sizeof(cplx) * n);
+    ocdi_project_intensity(o, o->tPF);
+  }
+  if (o->alg[PS] != 0) {
+    memcpy(o->tPS, x, sizeof(cplx) * n);
+    ocdi_apply_support(o, o->tPS);
+  }
+  if (o->alg[PSF] != 0) {
+    if (o->alg[PF] != 0) {
+      memcpy(o->tPSF, o->tPF, sizeof(cplx) * n);
+      ocdi_apply_support(o, o->tPSF);
+    } else {
+      memcpy(o->tPSF, x, sizeof(cplx) * n);
+      ocdi_project_intensity(o, o->tPSF);
+      ocdi_apply_support(o, o->tPSF);
+    }
+  }
+  for (size_t p = 0; p < n; p++) {
+    double vr = (1 + o->alg[PI_]) * x[p].re;
+    double vi = (1 + o->alg[PI_]) * x[p].im;
+    if (o->alg[PFS] != 0) { vr += o->alg[PFS] * o->tPFS[p].re; vi += o->alg[PFS] * o->tPFS[p].im; }
+    if (o->alg[PF] != 0) { vr += o->alg[PF] * o->tPF[p].re; vi += o->alg[PF] * o->tPF[p].im; }
+    if (o->alg[PS] != 0) { vr += o->alg[PS] * o->tPS[p].re; vi += o->alg[PS] * o->tPS[p].im; }
+    if (o->alg[PSF] != 0) { vr += o->alg[PSF] * o->tPSF[p].re; vi += o->alg[PSF] * o->tPSF[p].im; }
+    x[p].re = (real)vr;
+    x[p].im = (real)vi;
+  }
+  update_n_best(o);
+  return 1;
+}
+
+/* BaseCDI::apply_threshold: NADIA/src/BaseCDI.c++:477-492 (loops j < nx) */
+static void apply_threshold(ocdi *o, real *a, double threshold) {
+  double max = r_max(a, NPIX(o));
+  for (int i = 0; i < o->nx; i++)
+    for (int j = 0; j < o->nx; j++) {
+      size_t p = (size_t)i * o->ny + j;
+      if (a[p] < threshold * max) a[p] = 0.0;
+      else a[p] = 1.0;
+    }
+}
+
+/* BaseCDI::apply_shrinkwrap: NADIA/src/BaseCDI.c++:416-429 */
+void ocdi_apply_shrinkwrap(ocdi *o, double gauss_width, double threshold) {
+  real *recon = calloc_r(NPIX(o));
+  for (size_t p = 0; p < NPIX(o); p++) recon[p] = c_mag(o->complex[p]);
+  ocdi_convolve(o, recon, gauss_width, 4);
+  apply_threshold(o, recon, threshold);
+  ocdi_set_support(o, recon, 0);
+  free(recon);
+}
+
+void ocdi_get_support(ocdi *o, real *out) { memcpy(out, o->support, sizeof(real) * NPIX(o)); }
+void ocdi_get_intensity_sqrt(ocdi *o, real *out) { memcpy(out, o->intensity_sqrt, sizeof(real) * NPIX(o)); }
+
+/* PlanarCDI::initialise_estimate: NADIA/src/PlanarCDI.c++:39-62 (glibc rand;
+ * max sqrt(I) truncated to int) */
+static void planar_initialise_estimate(ocdi *o, int seed) {
+  srand(seed);
+  int max_value = (int)r_max(o->intensity_sqrt, NPIX(o));
+  for (size_t p = 0; p < NPIX(o); p++) {
+    if (!o->support[p]) { o->complex[p].re = 0; o->complex[p].im = 0; }
+    else {
+      /* the reference multiplies two ints (max_value*rand()); the product
+       * wraps modulo 2^32 on its target, reproduced here without UB */
+      int pr = (int)((unsigned)max_value * (unsigned)rand());
+      double r = o->support[p] * (pr / (double)RAND_MAX);
+      int pi = (int)((unsigned)max_value * (unsigned)rand());
+      double im = o->support[p] * (pi / (double)RAND_MAX);
+      o->complex[p].re = (real)r;
+      o->complex[p].im = (real)im;
+    }
+  }
+}
+
+/* FresnelCDI::initialise_estimate: NADIA/src/FresnelCDI.c++:152-201 */
+static void fresnel_initialise_estimate(ocdi *o, int seed) {
+  srand(seed);
+  cplx *rt = calloc_c(NPIX(o));
+  for (size_t p = 0; p < NPIX(o); p++) {
+    rt[p].re = (real)(1 - 0.2 * rand() / ((double)RAND_MAX));
+    rt[p].im = 0;
+  }
+  ocdi_set_transmission_function(o, rt, o->complex);
+  ocdi_apply_support(o, o->complex);
+  free(rt);
+}
+
+/* FresnelCDI_WF::initialise_estimate: NADIA/src/FresnelCDI_WF.c++:99-117 */
+static void wf_initialise_estimate(ocdi *o, int seed) {
+  srand(seed);
+  for (size_t p = 0; p < NPIX(o); p++) {
+    if (!o->support[p]) { o->complex[p].re = 0; o->complex[p].im = 0; }
+    else { o->complex[p].re = o->intensity_sqrt[p]; o->complex[p].im = 0; }
+  }
+}
+
+void ocdi_initialise_estimate(ocdi *o, int seed) {
+  if (o->kind == KIND_PLANAR) planar_initialise_estimate(o, seed);
+  else if (o->kind == KIND_FRESNEL) fresnel_initialise_estimate(o, seed);
+  else wf_initialise_estimate(o, seed);
+}
+
+/* PlanarCDI::get_intensity_autocorrelation: NADIA/src/PlanarCDI.c++:13-36 */
+void ocdi_get_intensity_autocorrelation(ocdi *o, real *autoc) {
+  cplx *t = calloc_c(NPIX(o));
+  for (size_t p = 0; p < NPIX(o); p++) {
+    double comp = (1.0 / sqrt(2.0)) * ((double)o->intensity_sqrt[p] * o->intensity_sqrt[p]);
+    t[p].re = (real)comp;
+    t[p].im = (real)comp;
+  }
+  or_fft2d(t, o->nx, o->ny, +1);
+  or_invert(t, o->nx, o->ny, 1);
+  for (size_t p = 0; p < NPIX(o); p++) autoc[p] = c_mag(t[p]);
+  free(t);
+}
+
+/* ---- FresnelCDI set-up ------------------------------------------------ */
+
+/* FresnelCDI::set_experimental_parameters: NADIA/src/FresnelCDI.c++:57-95 */
+void ocdi_fresnel_set_experimental_parameters(ocdi *o, double wl, double zfd, double zfs, double pixel) {
+  int nx = o->nx, ny = o->ny;
+  o->wavelength = wl; o->pixel = pixel; o->zfd = zfd; o->zfs = zfs;
+  double xm = (nx - 1) / 2.0, ym = (ny - 1) / 2.0;
+  double zsd = zfd - zfs;
+  double factor = pixel * pixel * M_PI / wl * ((1 / zfd) - (1 / zsd));
+  for (int i = 0; i < nx / 2.0; i++)
+    for (int j = 0; j < ny / 2.0; j++) {
+      double phi = factor * ((xm - i) * (xm - i) + (ym - j) * (ym - j));
+      o->coefficient[(size_t)i * (ny / 2) + j].re = (real)cos(phi);
+      o->coefficient[(size_t)i * (ny / 2) + j].im = (real)sin(phi);
+    }
+  if (!o->illum_at_sample) o->illum_at_sample = calloc_c(NPIX(o));
+  memcpy(o->illum_at_sample, o->illumination, sizeof(cplx) * NPIX(o));
+  ocdi_propagate_from_detector(o, o->illum_at_sample);
+}
+
+/* FresnelCDI::FresnelCDI: NADIA/src/FresnelCDI.c++:20-44 */
+void ocdi_fresnel_init(ocdi *o, const cplx *white_field, double wl, double zfd, double zfs,
+                       double pixel, double normalisation) {
+  o->norm = normalisation;
+  o->illumination = calloc_c(NPIX(o));
+  o->coefficient = calloc_c((size_t)(o->nx / 2) * (o->ny / 2));
+  memcpy(o->illumination, white_field, sizeof(cplx) * NPIX(o));
+  or_scale(o->illumination, NPIX(o), (real)o->norm);
+  ocdi_fresnel_set_experimental_parameters(o, wl, zfd, zfs, pixel);
+  ocdi_set_algorithm(o, ER);
+}
+
+/* FresnelCDI::set_norm: NADIA/src/FresnelCDI.c++:139-148 */
+void ocdi_fresnel_set_norm(ocdi *o, double nn) {
+  double old = o->norm;
+  o->norm = nn;
+  or_scale(o->illumination, NPIX(o), (real)(nn / old));
+  if (o->illum_at_sample) or_scale(o->illum_at_sample, NPIX(o), (real)(nn / old));
+}
+
+/* FresnelCDI::auto_set_norm: NADIA/src/FresnelCDI.c++:129-137 */
+void ocdi_fresnel_auto_set_norm(ocdi *o) {
+  real *t = calloc_r(NPIX(o));
+  for (size_t p = 0; p < NPIX(o); p++) t[p] = c_mag(o->illumination[p]);
+  double wf = r_sum(t, NPIX(o));
+  double in = r_sum(o->intensity_sqrt, NPIX(o));
+  free(t);
+  ocdi_fresnel_set_norm(o, in / wf);
+}
+
+void ocdi_fresnel_get_illumination_at_sample(ocdi *o, cplx *out) {
+  memcpy(out, o->illum_at_sample, sizeof(cplx) * NPIX(o));
+}
+void ocdi_fresnel_get_illumination(ocdi *o, cplx *out) {
+  memcpy(out, o->illumination, sizeof(cplx) * NPIX(o));
+}
+
+/* ---- FresnelCDI_WF set-up --------------------------------------------- */
+
+/* FresnelCDI_WF::FresnelCDI_WF: NADIA/src/FresnelCDI_WF.c++:17-67 */
+void ocdi_wf_init(ocdi *o, double wl, double z12, double z23, double pixel) {
+  int nx = o->nx, ny = o->ny;
+  o->wavelength = wl; o->z12 = z12; o->z23 = z23; o->pixel = pixel;
+  o->coefficient = calloc_c((size_t)(nx / 2) * (ny / 2));
+  double xm = (nx - 1) / 2.0, ym = (ny - 1) / 2.0;
+  double sx = wl * z23 / (pixel * nx), sy = wl * z23 / (pixel * ny);
+  double nrm = 1 / (sqrt((double)(nx * ny)));
+  for (int i = 0; i < nx / 2; i++)
+    for (int j = 0; j < ny / 2; j++) {
+      double rho2 = pow(sx * (xm - i), 2) + pow(sy * (ym - j), 2);
+      double phi = rho2 / z12 + rho2 / z23;
+      phi *= M_PI / wl;
+      o->coefficient[(size_t)i * (ny / 2) + j].re = (real)(sin(phi) * nrm);
+      o->coefficient[(size_t)i * (ny / 2) + j].im = (real)(-cos(phi) * nrm);
+    }
+}
+
+/* FresnelCDI_WF::set_support: NADIA/src/FresnelCDI_WF.c++:177-196 (integer
+ * centre (nx-1)/2, square arrays, softened and left unnormalised) */
+void ocdi_wf_set_support(ocdi *o, double z_diameter, double size) {
+  int nx = o->nx, ny = o->ny;
+  double centre = (nx - 1) / 2;
+  double dz = (o->z12 / o->z23) * o->pixel;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      double i_ = i - centre, j_ = j - centre;
+      double r = dz * sqrt(i_ * i_ + j_ * j_);
+      if (r < size * z_diameter / 2.0) o->support[(size_t)i * ny + j] = 1;
+    }
+  ocdi_convolve(o, o->support, 3, 5);
+}
+
+/* Run n iterations, recording get_error() after each (the driver loop of
+ * NADIA/tools/CDI_reconstruction.c:371-439 without file output). */
+void ocdi_iterate_n(ocdi *o, int n, double *errors) {
+  for (int k = 0; k < n; k++) {
+    ocdi_iterate(o);
+    if (errors) errors[k] = o->current_error;
+  }
+}
+
+int oracle_real_size(void) { return (int)sizeof(real); }

@@ -1,0 +1,255 @@
+"""ctypes front end of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, and only as the checker.  Arrays are numpy, x-major (shape
+(nx, ny), element (x, y) at x*ny + y), the layout of NADIA's Complex_2D
+(NADIA/include/Complex_2D.h:179-194) and Double_2D (NADIA/include/Double_2D.h:121-143).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+ER, BIO, BOO, HIO, DM, SF, ASR, HPR, RAAR, CUSTOM = range(10)
+PSF, PFS, PS, PF, PI = range(5)
+PLANAR, FRESNEL, FRESNEL_WF = 0, 1, 2
+
+_libs = {}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _lib(double=True):
+    key = "d" if double else "f"
+    if key in _libs:
+        return _libs[key]
+    path = os.path.join(HERE, "liboracle_%s.so" % key)
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    sig = {
+        "or_fft2d": (None, [P, I, I, I]),
+        "or_invert": (None, [P, I, I, I]),
+        "ocdi_create": (P, [P, I, I, I, I]),
+        "ocdi_destroy": (None, [P]),
+        "ocdi_convolve": (None, [P, P, D, I]),
+        "ocdi_set_support": (None, [P, P, I]),
+        "ocdi_set_beam_stop": (None, [P, P]),
+        "ocdi_set_intensity": (None, [P, P]),
+        "ocdi_get_error": (D, [P]),
+        "ocdi_set_transmission_function": (None, [P, P, P]),
+        "ocdi_get_transmission_function": (None, [P, P, P]),
+        "ocdi_propagate_to_detector": (None, [P, P]),
+        "ocdi_propagate_from_detector": (None, [P, P]),
+        "ocdi_scale_intensity": (None, [P, P]),
+        "ocdi_apply_support": (None, [P, P]),
+        "ocdi_project_intensity": (None, [P, P]),
+        "ocdi_set_custom_algorithm": (None, [P] + [D] * 10),
+        "ocdi_set_algorithm": (None, [P, I]),
+        "ocdi_set_relaxation_parameter": (None, [P, D]),
+        "ocdi_get_coefficients": (None, [P, P, P]),
+        "ocdi_get_best_result": (I, [P, I, P, P]),
+        "ocdi_reset_best": (None, [P]),
+        "ocdi_iterate": (I, [P]),
+        "ocdi_iterate_n": (None, [P, I, P]),
+        "ocdi_apply_shrinkwrap": (None, [P, D, D]),
+        "ocdi_get_support": (None, [P, P]),
+        "ocdi_get_intensity_sqrt": (None, [P, P]),
+        "ocdi_initialise_estimate": (None, [P, I]),
+        "ocdi_get_intensity_autocorrelation": (None, [P, P]),
+        "ocdi_fresnel_init": (None, [P, P, D, D, D, D, D]),
+        "ocdi_fresnel_set_experimental_parameters": (None, [P, D, D, D, D]),
+        "ocdi_fresnel_set_norm": (None, [P, D]),
+        "ocdi_fresnel_auto_set_norm": (None, [P]),
+        "ocdi_fresnel_get_illumination_at_sample": (None, [P, P]),
+        "ocdi_fresnel_get_illumination": (None, [P, P]),
+        "ocdi_wf_init": (None, [P, D, D, D, D]),
+        "ocdi_wf_set_support": (None, [P, D, D]),
+        "oracle_real_size": (I, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _libs[key] = lib
+    return lib
+
+
+def _ptr(a):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def fft2d(a, sign, double=True):
+    """Unnormalised in-place 2-D DFT (FFTW convention: sign -1 forward)."""
+    lib = _lib(double)
+    lib.or_fft2d(_ptr(a), a.shape[0], a.shape[1], sign)
+    return a
+
+
+def invert(a, scale=True, double=True):
+    """Complex_2D::invert (fftshift with optional 1/sqrt(N) scale), in place."""
+    lib = _lib(double)
+    lib.or_invert(_ptr(a), a.shape[0], a.shape[1], 1 if scale else 0)
+    return a
+
+
+class OracleCDI:
+    """One BaseCDI-derived object (PlanarCDI, FresnelCDI or FresnelCDI_WF)."""
+
+    def __init__(self, estimate, kind=PLANAR, n_best=0, double=True):
+        self.double = double
+        self.cdt = np.complex128 if double else np.complex64
+        self.rdt = np.float64 if double else np.float32
+        assert estimate.dtype == self.cdt and estimate.ndim == 2
+        self.lib = _lib(double)
+        self.nx, self.ny = estimate.shape
+        self.complex = estimate  # mutated in place, like BaseCDI::complex
+        self.h = self.lib.ocdi_create(_ptr(estimate), self.nx, self.ny, kind, n_best)
+        self.kind = kind
+        self.n_best = n_best
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.ocdi_destroy(self.h)
+            self.h = None
+
+    def _r(self, a):
+        return np.ascontiguousarray(a, dtype=self.rdt)
+
+    def _c(self, a):
+        return np.ascontiguousarray(a, dtype=self.cdt)
+
+    # -- BaseCDI ----------------------------------------------------------
+    def set_support(self, s, soften=False):
+        s = self._r(s)
+        self.lib.ocdi_set_support(self.h, _ptr(s), int(soften))
+
+    def set_intensity(self, i):
+        i = self._r(i)
+        self.lib.ocdi_set_intensity(self.h, _ptr(i))
+
+    def set_beam_stop(self, b):
+        b = self._r(b)
+        self.lib.ocdi_set_beam_stop(self.h, _ptr(b))
+
+    def set_algorithm(self, alg):
+        self.lib.ocdi_set_algorithm(self.h, alg)
+
+    def set_custom_algorithm(self, *m):
+        assert len(m) == 10
+        self.lib.ocdi_set_custom_algorithm(self.h, *[float(v) for v in m])
+
+    def set_relaxation_parameter(self, beta):
+        self.lib.ocdi_set_relaxation_parameter(self.h, beta)
+
+    def coefficients(self):
+        out = np.zeros(5)
+        alg = ctypes.c_int(0)
+        self.lib.ocdi_get_coefficients(self.h, _ptr(out), ctypes.byref(alg))
+        return out, alg.value
+
+    def iterate(self):
+        return self.lib.ocdi_iterate(self.h)
+
+    def iterate_n(self, n):
+        errs = np.zeros(n)
+        self.lib.ocdi_iterate_n(self.h, n, _ptr(errs))
+        return errs
+
+    def get_error(self):
+        return self.lib.ocdi_get_error(self.h)
+
+    def apply_shrinkwrap(self, gauss_width=1.5, threshold=0.1):
+        self.lib.ocdi_apply_shrinkwrap(self.h, gauss_width, threshold)
+
+    def get_support(self):
+        out = np.zeros((self.nx, self.ny), self.rdt)
+        self.lib.ocdi_get_support(self.h, _ptr(out))
+        return out
+
+    def get_intensity_sqrt(self):
+        out = np.zeros((self.nx, self.ny), self.rdt)
+        self.lib.ocdi_get_intensity_sqrt(self.h, _ptr(out))
+        return out
+
+    def convolve(self, arr, gauss_width, cutoff=4):
+        a = self._r(arr).copy()
+        self.lib.ocdi_convolve(self.h, _ptr(a), gauss_width, cutoff)
+        return a
+
+    def initialise_estimate(self, seed=0):
+        self.lib.ocdi_initialise_estimate(self.h, seed)
+
+    def apply_support(self, c):
+        self.lib.ocdi_apply_support(self.h, _ptr(c))
+
+    def scale_intensity(self, c):
+        self.lib.ocdi_scale_intensity(self.h, _ptr(c))
+
+    def project_intensity(self, c):
+        self.lib.ocdi_project_intensity(self.h, _ptr(c))
+
+    def propagate_to_detector(self, c):
+        self.lib.ocdi_propagate_to_detector(self.h, _ptr(c))
+
+    def propagate_from_detector(self, c):
+        self.lib.ocdi_propagate_from_detector(self.h, _ptr(c))
+
+    def get_best_result(self, index=0):
+        out = np.zeros((self.nx, self.ny), self.cdt)
+        err = ctypes.c_double(0)
+        ok = self.lib.ocdi_get_best_result(self.h, index, ctypes.byref(err), _ptr(out))
+        return (out, err.value) if ok else (None, None)
+
+    def reset_best(self):
+        self.lib.ocdi_reset_best(self.h)
+
+    def get_intensity_autocorrelation(self):
+        out = np.zeros((self.nx, self.ny), self.rdt)
+        self.lib.ocdi_get_intensity_autocorrelation(self.h, _ptr(out))
+        return out
+
+    # -- FresnelCDI -------------------------------------------------------
+    def fresnel_init(self, white_field, wavelength, zfd, zfs, pixel, norm=1.0):
+        wf = self._c(white_field)
+        self.lib.ocdi_fresnel_init(self.h, _ptr(wf), wavelength, zfd, zfs, pixel, norm)
+
+    def set_norm(self, n):
+        self.lib.ocdi_fresnel_set_norm(self.h, n)
+
+    def auto_set_norm(self):
+        self.lib.ocdi_fresnel_auto_set_norm(self.h)
+
+    def get_illumination_at_sample(self):
+        out = np.zeros((self.nx, self.ny), self.cdt)
+        self.lib.ocdi_fresnel_get_illumination_at_sample(self.h, _ptr(out))
+        return out
+
+    def get_illumination(self):
+        out = np.zeros((self.nx, self.ny), self.cdt)
+        self.lib.ocdi_fresnel_get_illumination(self.h, _ptr(out))
+        return out
+
+    def get_transmission_function(self, esw=None):
+        out = np.zeros((self.nx, self.ny), self.cdt)
+        e = None if esw is None else self._c(esw)
+        self.lib.ocdi_get_transmission_function(self.h, _ptr(out), None if e is None else _ptr(e))
+        return out
+
+    def set_transmission_function(self, T, esw=None):
+        T = self._c(T)
+        self.lib.ocdi_set_transmission_function(self.h, _ptr(T), None if esw is None else _ptr(esw))
+
+    # -- FresnelCDI_WF ----------------------------------------------------
+    def wf_init(self, wavelength, z12, z23, pixel):
+        self.lib.ocdi_wf_init(self.h, wavelength, z12, z23, pixel)
+
+    def wf_set_support(self, z_diameter, size=1.01):
+        self.lib.ocdi_wf_set_support(self.h, z_diameter, size)

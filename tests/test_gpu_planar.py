@@ -1,0 +1,245 @@
+"""GPU parity of the PlanarCDI path against the CPU oracle (FP64 <= 1e-12 for
+one iteration from the same state, <= 1e-9 on error trajectories)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_ITER = 1e-12   # one FP64 iteration, relative L2
+TOL_TRAJ = 1e-9    # FP64 per-iteration error trajectory, relative
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def small_state(synth, n=128, seed=3):
+    d = synth.planar_block(n=n, block=3 * n // 8, support=7 * n // 16, seed_amp=seed, seed_poisson=seed,
+                           seed_phase=seed + 1)
+    rng = np.random.default_rng(seed + 7)
+    # soft support: values in (0, 1] inside plus an exact-zero outside
+    d["soft_support"] = d["support"] * rng.uniform(0.3, 1.0, (n, n))
+    beam = np.ones((n, n))
+    beam[n // 2 - 3:n // 2 + 3, n // 2 - 3:n // 2 + 3] = 0.0
+    d["beam"] = beam
+    return d
+
+
+def make_pair(nd, oracle, d, double=True, soft=False, beam=False, n_best=0):
+    n = d["x0"].shape[0]
+    cdt = np.complex128 if double else np.complex64
+    est = nd.Complex2D(n, n, double=double, data=d["x0"].astype(cdt))
+    g = nd.PlanarCDI(est, n_best=n_best)
+    xo = d["x0"].astype(np.complex128).copy()
+    o = oracle.OracleCDI(xo, oracle.PLANAR, n_best=n_best)
+    sup = d["soft_support"] if soft else d["support"]
+    for obj in (g, o):
+        obj.set_support(sup)
+        obj.set_intensity(d["intensity"])
+        if beam:
+            obj.set_beam_stop(d["beam"])
+    return est, g, xo, o
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (128, 256), (96, 96), (384, 384), (2048, 32), (24, 768)])
+def test_fft_fp64_matches_oracle(nd, oracle, shape):
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    for sign, fn in ((-1, "perform_forward_fft"), (+1, "perform_backward_fft")):
+        f = nd.Complex2D(*shape, double=True, data=a)
+        getattr(f, fn)()
+        b = a.copy()
+        oracle.fft2d(b, sign)
+        assert rel(f.get(), b) < 1e-13, (shape, sign)
+
+
+@pytest.mark.parametrize("shape", [(256, 256), (2048, 2048), (768, 384)])
+def test_fft_fp32_accuracy(nd, shape):
+    rng = np.random.default_rng(2)
+    a = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    f = nd.Complex2D(*shape, double=False, data=a)
+    f.perform_forward_fft()
+    ref = np.fft.fft2(a.astype(np.complex128))
+    # FP32 FFT error grows like eps*log2(N)
+    assert rel(f.get(), ref) < 2e-6
+
+
+def test_invert_matches_oracle(nd, oracle):
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((64, 128)) + 1j * rng.standard_normal((64, 128))
+    for scale in (False, True):
+        f = nd.Complex2D(64, 128, data=a)
+        f.invert(scale)
+        b = a.copy()
+        oracle.invert(b, scale)
+        assert rel(f.get(), b) < 1e-15
+
+
+ALGS = ["ER", "BIO", "BOO", "HIO", "DM", "SF", "ASR", "HPR", "RAAR"]
+
+
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+@pytest.mark.parametrize("alg", ALGS)
+def test_one_iteration_fp64(nd, oracle, synth, alg, path):
+    d = small_state(synth)
+    est, g, xo, o = make_pair(nd, oracle, d, soft=True, beam=True)
+    g.set_path(path)
+    a = nd.get_alg_from_name(alg)
+    g.set_algorithm(a)
+    o.set_algorithm(a)
+    np.testing.assert_array_equal(g.get_coefficients()[0], o.coefficients()[0])
+    errs = g.iterate(1)
+    o.iterate()
+    assert rel(est.get(), xo) <= TOL_ITER, alg
+    assert abs(errs[0] - o.get_error()) <= TOL_ITER * abs(o.get_error())
+    assert g.get_error() == errs[0]
+
+
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+def test_custom_algorithm_and_zero_estimate(nd, oracle, synth, path):
+    """Custom m1..m10 with all five terms nonzero; zero estimate hits the
+    |y| = 0 branch of set_mag (y <- (sqrt I, 0))."""
+    d = small_state(synth, n=64)
+    d["x0"] = np.zeros_like(d["x0"])
+    est, g, xo, o = make_pair(nd, oracle, d, soft=True)
+    g.set_path(path)
+    m = (0.3, 0.5, 1.0, 0.2, 0.4, 0.1, 0.05, 0.2, 0.3, 0.1)
+    g.set_custom_algorithm(*m)
+    o.set_custom_algorithm(*m)
+    for _ in range(3):
+        e = g.iterate(1)[0]
+        o.iterate()
+        assert abs(e - o.get_error()) <= TOL_ITER * abs(o.get_error())
+    assert rel(est.get(), xo) <= TOL_ITER
+
+
+def test_relaxation_parameter_quirk(nd, oracle, synth):
+    """set_relaxation_parameter after set_algorithm does nothing
+    (NADIA/src/BaseCDI.c++:194-195)."""
+    d = small_state(synth, n=64)
+    est, g, xo, o = make_pair(nd, oracle, d)
+    for obj in (g, o):
+        obj.set_algorithm(nd.HIO)
+        obj.set_relaxation_parameter(0.5)
+    np.testing.assert_array_equal(g.get_coefficients()[0], o.coefficients()[0])
+    assert g.get_coefficients()[0][nd.PF] == -0.9
+    for obj in (g, o):
+        obj.set_relaxation_parameter(0.5)
+        obj.set_algorithm(nd.RAAR)
+    np.testing.assert_array_equal(g.get_coefficients()[0], o.coefficients()[0])
+
+
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+def test_er_trajectory_200_fp64(nd, oracle, synth, path):
+    """200 ER iterations on the 256^2 centred-block workload: per-iteration
+    get_error() within 1e-9 of the oracle's."""
+    d = synth.planar_block()
+    est, g, xo, o = make_pair(nd, oracle, d)
+    g.set_path(path)
+    eg = g.iterate(200)
+    eo = o.iterate_n(200)
+    r = np.abs(eg - eo) / np.abs(eo)
+    assert r.max() <= TOL_TRAJ, r.max()
+    assert rel(est.get(), xo) <= 1e-9
+
+
+def test_hio_er_workload_trajectory_fp64(nd, oracle, synth):
+    """200 HIO (beta 0.9) then 50 ER on the 256^2 centred-block workload."""
+    d = synth.planar_block()
+    est, g, xo, o = make_pair(nd, oracle, d)
+    for obj in (g, o):
+        obj.set_algorithm(nd.HIO)
+    eg = list(g.iterate(200))
+    eo = list(o.iterate_n(200))
+    for obj in (g, o):
+        obj.set_algorithm(nd.ER)
+    eg += list(g.iterate(50))
+    eo += list(o.iterate_n(50))
+    eg, eo = np.array(eg), np.array(eo)
+    r = np.abs(eg - eo) / np.abs(eo)
+    assert r.max() <= TOL_TRAJ, r.max()
+
+
+def test_shrinkwrap_fp64(nd, oracle, synth):
+    d = small_state(synth, n=128)
+    est, g, xo, o = make_pair(nd, oracle, d)
+    for obj in (g, o):
+        obj.set_algorithm(nd.HIO)
+    g.iterate(10)
+    o.iterate_n(10)
+    for sigma, thr in ((3.0, 0.1), (1.5, 0.2)):
+        g.apply_shrinkwrap(sigma, thr)
+        o.apply_shrinkwrap(sigma, thr)
+        sg, so = g.get_support(), o.get_support()
+        assert np.array_equal(sg, so)
+        assert set(np.unique(sg)) <= {0.0, 1.0}
+        g.iterate(5)
+        o.iterate_n(5)
+    assert rel(est.get(), xo) <= 1e-10
+
+
+def test_soft_support_set_support_soften(nd, oracle, synth):
+    d = small_state(synth, n=64)
+    est, g, xo, o = make_pair(nd, oracle, d)
+    g.set_support(d["soft_support"] * 3.0, soften=True)
+    o.set_support(d["soft_support"] * 3.0, True)
+    assert rel(g.get_support(), o.get_support()) <= 1e-14
+
+
+def test_initialise_estimate_planar(nd, oracle, synth):
+    d = small_state(synth, n=64)
+    est, g, xo, o = make_pair(nd, oracle, d, soft=True)
+    g.initialise_estimate(7)
+    o.initialise_estimate(7)
+    assert rel(est.get(), xo) <= 1e-15
+
+
+def test_n_best(nd, oracle, synth):
+    d = small_state(synth, n=64)
+    est, g, xo, o = make_pair(nd, oracle, d, n_best=3)
+    for obj in (g, o):
+        obj.set_algorithm(nd.HIO)
+    g.iterate(6)
+    o.iterate_n(6)
+    for i in range(3):
+        bg, eg = g.get_best_result(i)
+        bo, eo = o.get_best_result(i)
+        assert abs(eg - eo) <= 1e-12 * abs(eo)
+        assert rel(bg, bo) <= 1e-12
+
+
+def test_autocorrelation(nd, oracle, synth):
+    d = small_state(synth, n=64)
+    est, g, xo, o = make_pair(nd, oracle, d)
+    assert rel(g.get_intensity_autocorrelation(), o.get_intensity_autocorrelation()) <= 1e-13
+
+
+def test_public_operators(nd, oracle, synth):
+    """propagate_to/from_detector, scale_intensity, project_intensity and
+    apply_support on a separate field."""
+    d = small_state(synth, n=64)
+    est, g, xo, o = make_pair(nd, oracle, d, soft=True, beam=True)
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((64, 64)) + 1j * rng.standard_normal((64, 64))
+    for name in ("propagate_to_detector", "propagate_from_detector", "scale_intensity", "project_intensity",
+                 "apply_support"):
+        f = nd.Complex2D(64, 64, data=a)
+        getattr(g, name)(f)
+        b = a.copy()
+        getattr(o, name)(b)
+        assert rel(f.get(), b) <= 1e-13, name
+        if name in ("scale_intensity", "project_intensity"):
+            assert abs(g.get_error() - o.get_error()) <= 1e-12 * abs(o.get_error())
+
+
+def test_fp32_against_fp64_oracle(nd, oracle, synth):
+    """FP32 bulk arithmetic with FP64 error sums: one iteration within 1e-5
+    of the FP64 oracle, and the 256^2 HIO/ER trajectory close in error."""
+    d = synth.planar_block()
+    est, g, xo, o = make_pair(nd, oracle, d, double=False)
+    for obj in (g, o):
+        obj.set_algorithm(nd.HIO)
+    e1 = g.iterate(1)[0]
+    o.iterate()
+    assert rel(est.get().astype(np.complex128), xo) <= 1e-5
+    assert abs(e1 - o.get_error()) <= 1e-5 * o.get_error()
