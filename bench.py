@@ -78,6 +78,11 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=40)
     ap.add_argument("--cpu-sample", type=int, default=12, help="oracle HIO iterations timed on the host")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp32-only", action="store_true",
+                    help="run only the FP32 fused iterations (for profiler runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
+                    help="instead of the benchmark, run K device copies of a 2048^2 complex64 field "
+                         "(32 MiB read + 32 MiB written each) to calibrate profiler byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -86,6 +91,15 @@ def main():
 
     nd = importlib.import_module("nadia-coecxs_amd")
     synth = importlib.import_module("nadia-coecxs_amd.synth")
+    if args.copy_calibration:
+        a = nd.Complex2D(N, N, double=False)
+        b = nd.Complex2D(N, N, double=False)
+        for _ in range(args.copy_calibration):
+            b.copy(a)
+        nd.synchronize()
+        print(json.dumps({"copy_calibration": args.copy_calibration, "bytes_read_per_copy": N * N * 8,
+                          "bytes_written_per_copy": N * N * 8}))
+        return
     t_gen = time.perf_counter()
     data = synth.planar_star(n=N)
     t_gen = time.perf_counter() - t_gen
@@ -122,6 +136,9 @@ def main():
         "iteration_achieved_GBs": round(b32["iter"] * N * N * value / 1e9, 1),
     }
     del p, est
+    if args.fp32_only:
+        print(json.dumps({"fp32_fused_iter_s": round(value, 2), "roofline": roofline}))
+        return
 
     # ---- FP64 plain (reference data flow) and FP64 fused ----------------
     est64, p64 = setup(nd, data, True, nd.PATH_PLAIN)

@@ -1,0 +1,63 @@
+"""The C++ drop-in classes (include/nadia/*.h, libNADIA[d]_amd.so) driven by
+an example program written like NADIA's examples/PlanarCDI_example.c,
+checked against the CPU oracle running the same loop."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CPP = os.path.join(ROOT, "nadia-coecxs_amd", "cpp")
+
+
+def run_example(binary, tmp_path, d, er1, hio, er2, sw):
+    n = d["x0"].shape[0]
+    d["intensity"].astype(np.float64).tofile(tmp_path / "intensity.raw")
+    d["support"].astype(np.float64).tofile(tmp_path / "support.raw")
+    d["x0"].astype(np.complex128).tofile(tmp_path / "x0.raw")
+    out = subprocess.run([os.path.join(CPP, binary), str(tmp_path), str(n), str(n), str(er1), str(hio), str(er2),
+                          str(sw)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    errs = [float(l.split()[-1]) for l in out.stdout.splitlines() if l.startswith("iteration ")]
+    best = [float(l.split()[-1]) for l in out.stdout.splitlines() if l.startswith("best error")][0]
+    x = np.fromfile(tmp_path / "out.raw", dtype=np.complex128).reshape(n, n)
+    return np.array(errs), x, best
+
+
+def oracle_loop(oracle, d, er1, hio, er2, sw):
+    x = d["x0"].astype(np.complex128).copy()
+    o = oracle.OracleCDI(x, oracle.PLANAR, n_best=4)
+    o.set_support(d["support"])
+    o.set_intensity(d["intensity"])
+    errs = []
+    it = 0
+    for alg, cnt in ((oracle.ER, er1), (oracle.HIO, hio), (oracle.ER, er2)):
+        o.set_algorithm(alg)
+        for _ in range(cnt):
+            o.iterate()
+            errs.append(o.get_error())
+            if sw > 0 and it % sw == sw - 1:
+                o.apply_shrinkwrap(2.0, 0.1)
+            it += 1
+    return np.array(errs), x, o.get_best_result(0)[1]
+
+
+def test_cpp_planar_example_fp64(oracle, synth, tmp_path):
+    d = synth.planar_block(n=128, block=48, support=56)
+    eg, xg, bg = run_example("planar_exampled", tmp_path, d, 20, 20, 20, 25)
+    eo, xo, bo = oracle_loop(oracle, d, 20, 20, 20, 25)
+    assert len(eg) == 60
+    assert (np.abs(eg - eo) / eo).max() <= 1e-9
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-9
+    assert abs(bg - bo) <= 1e-9 * bo
+
+
+def test_cpp_planar_example_fp32(oracle, synth, tmp_path):
+    d = synth.planar_block(n=128, block=48, support=56)
+    eg, xg, bg = run_example("planar_example", tmp_path, d, 20, 0, 0, 0)
+    eo, xo, bo = oracle_loop(oracle, d, 20, 0, 0, 0)
+    assert (np.abs(eg - eo) / eo).max() <= 1e-4
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) <= 1e-4

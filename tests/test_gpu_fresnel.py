@@ -1,0 +1,156 @@
+"""GPU parity of FresnelCDI and FresnelCDI_WF against the CPU oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL_ITER = 1e-12
+TOL_TRAJ = 1e-9
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def fresnel_pair(nd, oracle, synth, n=128, double=True, path=0):
+    d = synth.fresnel_config(n=n, pupil_diameter=160e-6 * n / 1024, disk_radius=20 * n / 1024 + 2,
+                             ndisks=4, support=500)
+    cdt = np.complex128 if double else np.complex64
+    est = nd.Complex2D(n, n, double=double)
+    g = nd.FresnelCDI(est, d["white_field"].astype(cdt), d["wavelength"], d["zfd"], d["zfs"], d["pixel"], 1.0)
+    g.set_path(path)
+    xo = np.zeros((n, n), np.complex128)
+    o = oracle.OracleCDI(xo, oracle.FRESNEL)
+    o.fresnel_init(d["white_field"], d["wavelength"], d["zfd"], d["zfs"], d["pixel"], 1.0)
+    for obj in (g, o):
+        obj.set_support(d["support"])
+        obj.set_intensity(d["intensity"])
+    return d, est, g, xo, o
+
+
+def test_fresnel_illumination_at_sample(nd, oracle, synth):
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
+    assert rel(g.get_illumination_at_sample(), o.get_illumination_at_sample()) <= 1e-13
+
+
+def test_fresnel_propagators(nd, oracle, synth):
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((128, 128)) + 1j * rng.standard_normal((128, 128))
+    for name in ("propagate_to_detector", "propagate_from_detector", "scale_intensity", "project_intensity"):
+        f = nd.Complex2D(128, 128, data=a)
+        getattr(g, name)(f)
+        b = a.copy()
+        getattr(o, name)(b)
+        assert rel(f.get(), b) <= 1e-13, name
+
+
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+@pytest.mark.parametrize("alg", ["ER", "HIO", "RAAR", "DM"])
+def test_fresnel_one_iteration(nd, oracle, synth, alg, path):
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth, path=path)
+    g.initialise_estimate(0)
+    o.initialise_estimate(0)
+    assert rel(est.get(), xo) <= 1e-14
+    a = nd.get_alg_from_name(alg)
+    g.set_algorithm(a)
+    o.set_algorithm(a)
+    e = g.iterate(1)[0]
+    o.iterate()
+    assert rel(est.get(), xo) <= TOL_ITER
+    assert abs(e - o.get_error()) <= TOL_ITER * o.get_error()
+
+
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+def test_fresnel_er_trajectory(nd, oracle, synth, path):
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth, path=path)
+    g.initialise_estimate(1)
+    o.initialise_estimate(1)
+    eg = g.iterate(60)
+    eo = o.iterate_n(60)
+    assert (np.abs(eg - eo) / eo).max() <= TOL_TRAJ
+    assert rel(est.get(), xo) <= 1e-9
+
+
+def test_fresnel_transmission_and_norm(nd, oracle, synth):
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
+    rng = np.random.default_rng(2)
+    T = np.exp(-0.2 * rng.uniform(size=(128, 128)) + 1j * rng.uniform(size=(128, 128)))
+    g.set_transmission_function(T)
+    o.set_transmission_function(T)
+    assert rel(est.get(), xo) <= 1e-14
+    assert rel(g.get_transmission_function(), o.get_transmission_function()) <= 1e-13
+    g.auto_set_norm()
+    o.auto_set_norm()
+    assert rel(g.get_illumination_at_sample(), o.get_illumination_at_sample()) <= 1e-12
+    e = g.iterate(3)
+    eo = o.iterate_n(3)
+    assert (np.abs(e - eo) / eo).max() <= TOL_TRAJ
+
+
+def test_fresnel_fp32(nd, oracle, synth):
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth, double=False)
+    g.initialise_estimate(0)
+    o.initialise_estimate(0)
+    e = g.iterate(5)
+    eo = o.iterate_n(5)
+    assert rel(est.get().astype(np.complex128), xo) <= 1e-4
+    assert (np.abs(e - eo) / eo).max() <= 1e-4
+
+
+WF = dict(wavelength=1.55e-10, z12=16.353e-3, z23=0.909, pixel=13.5e-6)
+
+
+def wf_pair(nd, oracle, synth, n=128, double=True, path=0):
+    d = synth.fresnel_config(n=n, pupil_diameter=160e-6 * n / 1024, ndisks=1)
+    cdt = np.complex128 if double else np.complex64
+    est = nd.Complex2D(n, n, double=double)
+    g = nd.FresnelCDI_WF(est, WF["wavelength"], WF["z12"], WF["z23"], WF["pixel"])
+    g.set_path(path)
+    xo = np.zeros((n, n), np.complex128)
+    o = oracle.OracleCDI(xo, oracle.FRESNEL_WF)
+    o.wf_init(WF["wavelength"], WF["z12"], WF["z23"], WF["pixel"])
+    g.set_support(d["pupil_diameter"])
+    o.wf_set_support(d["pupil_diameter"])
+    for obj in (g, o):
+        obj.set_intensity(d["white_field_intensity"])
+    return d, est, g, xo, o
+
+
+def test_wf_support_and_propagators(nd, oracle, synth):
+    d, est, g, xo, o = wf_pair(nd, oracle, synth)
+    assert rel(g.get_support(), o.get_support()) <= 1e-14
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((128, 128)) + 1j * rng.standard_normal((128, 128))
+    for name in ("propagate_to_detector", "propagate_from_detector"):
+        f = nd.Complex2D(128, 128, data=a)
+        getattr(g, name)(f)
+        b = a.copy()
+        getattr(o, name)(b)
+        assert rel(f.get(), b) <= 1e-12, name
+
+
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+def test_wf_iterations(nd, oracle, synth, path):
+    d, est, g, xo, o = wf_pair(nd, oracle, synth, path=path)
+    g.initialise_estimate(0)
+    o.initialise_estimate(0)
+    assert rel(est.get(), xo) <= 1e-15
+    e1 = g.iterate(1)[0]
+    o.iterate()
+    assert rel(est.get(), xo) <= TOL_ITER
+    assert abs(e1 - o.get_error()) <= TOL_ITER * o.get_error()
+    eg = g.iterate(49)
+    eo = o.iterate_n(49)
+    assert (np.abs(eg - eo) / eo).max() <= TOL_TRAJ
+    g.set_algorithm(nd.HIO)  # fixed algorithm: ignored like the reference
+    assert g.get_coefficients()[1] == nd.ER
+
+
+def test_wf_fp32(nd, oracle, synth):
+    d, est, g, xo, o = wf_pair(nd, oracle, synth, double=False)
+    g.initialise_estimate(0)
+    o.initialise_estimate(0)
+    eg = g.iterate(10)
+    eo = o.iterate_n(10)
+    assert (np.abs(eg - eo) / eo).max() <= 1e-4
