@@ -182,7 +182,16 @@ int nadia_wf_setup(nadia_recon* r, double wavelength, double zone_focal_length,
 /* set_support(z_diameter, size) (src/FresnelCDI_WF.c++:177-196) */
 int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size);
 
-/* ---- measurement ---- */
+/* ---- tuning and measurement ---- */
+
+/* Tile variants of the fused passes (no counterpart in the reference).  By
+ * default an autotuner times the variants on the first fused iterate() of
+ * each (precision, kind, size) and keeps the fastest for the process; every
+ * variant computes bit-identical results.  Setting both to -1 returns to the
+ * autotuner; get returns -1 before the first fused iterate(). */
+int nadia_recon_set_tile_variants(nadia_recon* r, int col_variant, int row_variant);
+int nadia_recon_get_tile_variants(const nadia_recon* r, int* col_variant, int* row_variant);
+
 
 /* Record the device time of every fused pass launched by iterate() with HIP
  * events on the library stream (on = 1); nadia_recon_pass_times returns, per

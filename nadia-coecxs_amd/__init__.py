@@ -99,6 +99,8 @@ def _sig(lib):
         "nadia_fresnel_set_transmission_function": (I, [P, P, P]),
         "nadia_wf_setup": (I, [P, D, D, D, D]),
         "nadia_wf_set_support": (I, [P, D, D]),
+        "nadia_recon_set_tile_variants": (I, [P, I, I]),
+        "nadia_recon_get_tile_variants": (I, [P, IP, IP]),
         "nadia_recon_set_timing": (I, [P, I]),
         "nadia_recon_pass_times": (I, [P, DP, LP]),
     }
@@ -325,6 +327,15 @@ class BaseCDI:
 
     def set_path(self, path):
         _check(library().nadia_recon_set_path(self.h, int(path)))
+
+    def set_tile_variants(self, col_variant, row_variant):
+        """Fix the fused passes' tile variants (-1, -1: back to the autotuner)."""
+        _check(library().nadia_recon_set_tile_variants(self.h, int(col_variant), int(row_variant)))
+
+    def get_tile_variants(self):
+        c, r = ctypes.c_int(), ctypes.c_int()
+        _check(library().nadia_recon_get_tile_variants(self.h, ctypes.byref(c), ctypes.byref(r)))
+        return c.value, r.value
 
     def set_timing(self, on=True):
         _check(library().nadia_recon_set_timing(self.h, int(bool(on))))

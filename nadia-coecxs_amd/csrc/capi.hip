@@ -124,6 +124,8 @@ void* twiddles(DeviceCtx* c, int n, int prec, bool row) {
 
 enum { A_PSF = 0, A_PFS = 1, A_PS = 2, A_PF = 3, A_PI = 4 };
 
+int alloc(void** p, size_t bytes);
+
 }  // namespace
 
 struct nadia_field {
@@ -171,6 +173,9 @@ struct nadia_recon {
   // n best
   std::vector<void*> best;
   std::vector<double> best_err;
+  // tile variants of the fused passes (-1: not chosen yet)
+  int col_v, row_v;
+  bool tile_fixed;  // set by the caller, not the autotuner
   // timing
   bool timing;
   double pass_ms[2];
@@ -197,7 +202,7 @@ int fft2d_T(DeviceCtx* c, void* buf, int nx, int ny, int dir) {
   ra.dst = buf;
   ra.dir = dir;
   ra.scale = 1.0;
-  launch_row<T>(ra, c->stream);
+  launch_row<T>(ra, 0, c->stream);
   LAUNCHCHK();
   ColArgs ca{};
   ca.mode = COL_FFT;
@@ -207,7 +212,7 @@ int fft2d_T(DeviceCtx* c, void* buf, int nx, int ny, int dir) {
   ca.buf = buf;
   ca.dir = dir;
   ca.scale = 1.0;
-  launch_col<T>(ca, 1, c->stream);
+  launch_col<T>(ca, 1, 0, c->stream);
   LAUNCHCHK();
   return 0;
 }
@@ -418,6 +423,7 @@ int fused_prepare(nadia_recon* r, bool two) {
   void* twy = twiddles(r->ctx, r->ny, r->prec, true);
   if (!twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
   const long long n = r->npix;
+  const int rv = r->row_v < 0 ? 0 : r->row_v;
   if (!r->w_valid || r->w_version != r->est->version) {
     RowArgs ra{};
     ra.mode = ROW_FFT;
@@ -433,7 +439,7 @@ int fused_prepare(nadia_recon* r, bool two) {
     } else {
       ra.dir = -1;
     }
-    launch_row<T>(ra, S_(r));
+    launch_row<T>(ra, rv, S_(r));
     LAUNCHCHK();
     r->w_valid = true;
     r->w_version = r->est->version;
@@ -451,7 +457,7 @@ int fused_prepare(nadia_recon* r, bool two) {
     ra.dst = W2;
     ra.dir = -1;
     ra.scale = 1.0;
-    launch_row<T>(ra, S_(r));
+    launch_row<T>(ra, rv, S_(r));
     LAUNCHCHK();
     r->w2_valid = true;
     r->w2_version = r->est->version;
@@ -460,137 +466,222 @@ int fused_prepare(nadia_recon* r, bool two) {
   return 0;
 }
 
-// Fused iterations of PlanarCDI / FresnelCDI (ER and the generic algorithms).
+bool two_fields(const nadia_recon* r) {
+  return r->kind != NADIA_FRESNEL_WF && r->algorithm != NADIA_ER && r->alg[A_PFS] != 0;
+}
+
+// Queue one fused iteration on the buffers X (estimate) and W (work fields),
+// with error slot `slot`, column/row tile variants cv/rv.  PlanarCDI and
+// FresnelCDI: column pass (modulus projection) + row pass (update).
+// FresnelCDI_WF: four passes and a final error reduction.
 template <typename T>
-int iterate_fused_pf(nadia_recon* r, int n, double* slots) {
-  const bool two = r->algorithm != NADIA_ER && r->alg[A_PFS] != 0;
-  int rc;
-  if ((rc = fused_prepare<T>(r, two))) return rc;
+int fused_iteration(nadia_recon* r, void* X, void* W, double* slot, int cv, int rv, std::vector<Ev>* evs) {
   void* twy = twiddles(r->ctx, r->ny, r->prec, true);
   void* twx = twiddles(r->ctx, r->nx, r->prec, false);
   if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
-  const int G = r->ny / col_tile<T>(r->nx);
-  // which projection's error get_error() reports: the last one the reference
-  // evaluates (P_F x when a PF or PSF term exists, else P_F P_S x)
-  const int err_field = (r->algorithm == NADIA_ER || r->alg[A_PF] != 0 || r->alg[A_PSF] != 0) ? 0 : 1;
+  const double sf = 1.0 / std::sqrt((double)r->nx * (double)r->ny);
   ColArgs ca{};
-  ca.mode = r->kind == NADIA_FRESNEL ? COL_FRESNEL : COL_PLANAR;
   ca.nx = r->nx;
   ca.ny = r->ny;
   ca.tw = twx;
-  ca.buf = r->W;
+  ca.buf = W;
   ca.field_stride = r->npix;
   ca.q = r->Q;
   ca.ill = r->L;
-  ca.sf = 1.0 / std::sqrt((double)r->nx * (double)r->ny);
+  ca.sf = sf;
   ca.phase_factor = r->phase_factor;
   ca.xmid = (r->nx - 1) / 2.0;
   ca.ymid = (r->ny - 1) / 2.0;
   ca.partials = r->partials;
   RowArgs ra{};
-  ra.mode = ROW_UPDATE;
   ra.nrows = r->nx;
   ra.ny = r->ny;
   ra.tw = twy;
-  ra.src = r->W;
-  ra.dst = r->W;
-  void* W2 = (char*)r->W + r->npix * r->esz;
-  ra.src2 = two ? W2 : nullptr;
-  ra.dst2 = two ? W2 : nullptr;
-  ra.x = r->est->d;
+  ra.src = W;
+  ra.dst = W;
+  ra.x = X;
   ra.s = r->S;
-  ra.er = r->algorithm == NADIA_ER;
-  ra.a = r->alg[A_PSF];
-  ra.b = r->alg[A_PFS];
-  ra.c = r->alg[A_PS];
-  ra.d = r->alg[A_PF];
-  ra.e = r->alg[A_PI];
-  ra.post = ca.sf;
-  ra.fin_partials = r->partials + 2LL * G * err_field;
-  ra.fin_n = G;
-  std::vector<Ev> evs;
-  for (int k = 0; k < n; ++k) {
-    if ((rc = record_start(r, evs, 0))) return rc;
-    launch_col<T>(ca, two ? 2 : 1, S_(r));
-    if ((rc = record_stop(r, evs))) return rc;
+  ra.post = sf;
+  int rc;
+  if (r->kind != NADIA_FRESNEL_WF) {
+    const bool two = two_fields(r);
+    const int G = r->ny / col_tile<T>(r->nx, cv);
+    // get_error() reports the last projection the reference evaluates: P_F x
+    // when a PF or PSF term exists, else P_F P_S x
+    const int err_field = (r->algorithm == NADIA_ER || r->alg[A_PF] != 0 || r->alg[A_PSF] != 0) ? 0 : 1;
+    ca.mode = r->kind == NADIA_FRESNEL ? COL_FRESNEL : COL_PLANAR;
+    ra.mode = ROW_UPDATE;
+    void* W2 = (char*)W + r->npix * r->esz;
+    ra.src2 = two ? W2 : nullptr;
+    ra.dst2 = two ? W2 : nullptr;
+    ra.er = r->algorithm == NADIA_ER;
+    ra.a = r->alg[A_PSF];
+    ra.b = r->alg[A_PFS];
+    ra.c = r->alg[A_PS];
+    ra.d = r->alg[A_PF];
+    ra.e = r->alg[A_PI];
+    ra.fin_partials = r->partials + 2LL * G * err_field;
+    ra.fin_n = G;
+    ra.err_out = slot;
+    if (evs && (rc = record_start(r, *evs, 0))) return rc;
+    launch_col<T>(ca, two ? 2 : 1, cv, S_(r));
+    if (evs && (rc = record_stop(r, *evs))) return rc;
     LAUNCHCHK();
-    ra.err_out = slots + k;
-    if ((rc = record_start(r, evs, 1))) return rc;
-    launch_row<T>(ra, S_(r));
-    if ((rc = record_stop(r, evs))) return rc;
+    if (evs && (rc = record_start(r, *evs, 1))) return rc;
+    launch_row<T>(ra, rv, S_(r));
+    if (evs && (rc = record_stop(r, *evs))) return rc;
     LAUNCHCHK();
+    return 0;
   }
+  ca.wf_sx = r->wf_sx;
+  ca.wf_sy = r->wf_sy;
+  ca.wf_z12 = r->z12;
+  ca.wf_z23 = r->z23;
+  ca.wf_k = r->wf_k;
+  ca.wf_norm = r->wf_norm;
+  ColArgs c3 = ca;
+  ca.mode = COL_WF1;
+  c3.mode = COL_WF3;
+  ra.mode = ROW_WF2;
+  RowArgs r4 = ra;
+  r4.mode = ROW_WF4;
+  r4.q = r->Q;
+  r4.partials = r->partials;
+  const int nb = row_blocks<T>(r->nx, r->ny, rv);
+  if (evs && (rc = record_start(r, *evs, 0))) return rc;
+  launch_col<T>(ca, 1, cv, S_(r));
+  if (evs && (rc = record_stop(r, *evs))) return rc;
+  if (evs && (rc = record_start(r, *evs, 1))) return rc;
+  launch_row<T>(ra, rv, S_(r));
+  if (evs && (rc = record_stop(r, *evs))) return rc;
+  if (evs && (rc = record_start(r, *evs, 0))) return rc;
+  launch_col<T>(c3, 1, cv, S_(r));
+  if (evs && (rc = record_stop(r, *evs))) return rc;
+  if (evs && (rc = record_start(r, *evs, 1))) return rc;
+  launch_row<T>(r4, rv, S_(r));
+  if (evs && (rc = record_stop(r, *evs))) return rc;
+  k_final_sum2(r->partials, nb, slot, nullptr, S_(r));
+  LAUNCHCHK();
+  return 0;
+}
+
+// ---- autotuner -------------------------------------------------------------
+// Picks the column and row tile variants of the fused passes for one
+// (precision, kind, nx, ny, field count) by timing real fused iterations on
+// scratch copies of the estimate and work fields (the reconstruction's own
+// state is not touched).  Choices are cached for the life of the process.
+// NADIA_TUNE=0 disables it (default variants); NADIA_COL_VARIANT /
+// NADIA_ROW_VARIANT force a choice.
+struct TuneKey {
+  int prec, kind, nx, ny, two;
+  bool operator<(const TuneKey& o) const {
+    if (prec != o.prec) return prec < o.prec;
+    if (kind != o.kind) return kind < o.kind;
+    if (nx != o.nx) return nx < o.nx;
+    if (ny != o.ny) return ny < o.ny;
+    return two < o.two;
+  }
+};
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+
+int env_int(const char* name, int dflt) {
+  const char* v = getenv(name);
+  return (v && *v) ? atoi(v) : dflt;
+}
+
+template <typename T>
+int time_variant(nadia_recon* r, void* X, void* W, int cv, int rv, int reps, float* ms) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  int rc = 0;
+  HIPCHK(hipEventRecord(a, S_(r)));
+  for (int k = 0; k < reps && !rc; ++k) rc = fused_iteration<T>(r, X, W, r->scal + 12, cv, rv, nullptr);
+  HIPCHK(hipEventRecord(b, S_(r)));
+  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipEventElapsedTime(ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+template <typename T>
+int autotune(nadia_recon* r) {
+  if (r->tile_fixed && r->col_v >= 0 && r->row_v >= 0) return 0;
+  const int fc = env_int("NADIA_COL_VARIANT", -1), fr = env_int("NADIA_ROW_VARIANT", -1);
+  if (fc >= 0 || fr >= 0 || env_int("NADIA_TUNE", 1) == 0 || (!tuned_size(r->nx) && !tuned_size(r->ny))) {
+    r->col_v = (fc >= 0 && fc < kColVariants) ? fc : 0;
+    r->row_v = (fr >= 0 && fr < kRowVariants) ? fr : 0;
+    return 0;
+  }
+  const TuneKey key{r->prec, r->kind, r->nx, r->ny, two_fields(r) ? 1 : 0};
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      r->col_v = it->second.first;
+      r->row_v = it->second.second;
+      return 0;
+    }
+  }
+  void *X = nullptr, *W = nullptr;
+  int rc;
+  if ((rc = alloc(&X, r->npix * r->esz))) return rc;
+  if ((rc = alloc(&W, 2 * r->npix * r->esz))) {
+    (void)hipFree(X);
+    return rc;
+  }
+  HIPCHK(hipMemcpyAsync(X, r->est->d, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+  HIPCHK(hipMemcpyAsync(W, r->W, 2 * r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+  const int reps = 6;
+  int best_c = 0, best_r = 0;
+  float best = 1e30f, ms = 0;
+  for (int cv = 0; cv < kColVariants && !rc; ++cv) {
+    if ((rc = time_variant<T>(r, X, W, cv, 0, 2, &ms))) break;  // warm-up
+    if ((rc = time_variant<T>(r, X, W, cv, 0, reps, &ms))) break;
+    if (ms < best) {
+      best = ms;
+      best_c = cv;
+    }
+  }
+  best = 1e30f;
+  for (int rv = 0; rv < kRowVariants && !rc; ++rv) {
+    if ((rc = time_variant<T>(r, X, W, best_c, rv, 2, &ms))) break;
+    if ((rc = time_variant<T>(r, X, W, best_c, rv, reps, &ms))) break;
+    if (ms < best) {
+      best = ms;
+      best_r = rv;
+    }
+  }
+  (void)hipStreamSynchronize(S_(r));
+  (void)hipFree(X);
+  (void)hipFree(W);
+  if (rc) return rc;
+  r->col_v = best_c;
+  r->row_v = best_r;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_tuned[key] = std::make_pair(best_c, best_r);
+  return 0;
+}
+
+template <typename T>
+int iterate_fused(nadia_recon* r, int n, double* slots) {
+  int rc;
+  if (r->col_v < 0 || r->row_v < 0 || !r->tile_fixed) {
+    if ((rc = fused_prepare<T>(r, two_fields(r)))) return rc;
+    if ((rc = autotune<T>(r))) return rc;
+  }
+  if ((rc = fused_prepare<T>(r, two_fields(r)))) return rc;
+  std::vector<Ev> evs;
+  for (int k = 0; k < n; ++k)
+    if ((rc = fused_iteration<T>(r, r->est->d, r->W, slots + k, r->col_v, r->row_v, r->timing ? &evs : nullptr)))
+      return rc;
   r->est->version++;
   r->w_version = r->est->version;
-  if (two) {
+  if (two_fields(r)) {
     r->w2_version = r->est->version;
     r->w2_sversion = r->s_version;
   }
-  r->last_err = slots + (n - 1);
-  r->h_err_valid = false;
-  return collect_events(r, evs);
-}
-
-// Fused iterations of FresnelCDI_WF: four passes per iteration.
-template <typename T>
-int iterate_fused_wf(nadia_recon* r, int n, double* slots) {
-  int rc;
-  if ((rc = fused_prepare<T>(r, false))) return rc;
-  void* twy = twiddles(r->ctx, r->ny, r->prec, true);
-  void* twx = twiddles(r->ctx, r->nx, r->prec, false);
-  if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
-  ColArgs c1{};
-  c1.nx = r->nx;
-  c1.ny = r->ny;
-  c1.tw = twx;
-  c1.buf = r->W;
-  c1.field_stride = r->npix;
-  c1.sf = 1.0 / std::sqrt((double)r->nx * (double)r->ny);
-  c1.xmid = (r->nx - 1) / 2.0;
-  c1.ymid = (r->ny - 1) / 2.0;
-  c1.wf_sx = r->wf_sx;
-  c1.wf_sy = r->wf_sy;
-  c1.wf_z12 = r->z12;
-  c1.wf_z23 = r->z23;
-  c1.wf_k = r->wf_k;
-  c1.wf_norm = r->wf_norm;
-  ColArgs c3 = c1;
-  c1.mode = COL_WF1;
-  c3.mode = COL_WF3;
-  RowArgs r2{};
-  r2.mode = ROW_WF2;
-  r2.nrows = r->nx;
-  r2.ny = r->ny;
-  r2.tw = twy;
-  r2.src = r->W;
-  r2.dst = r->W;
-  r2.s = r->S;
-  RowArgs r4 = r2;
-  r4.mode = ROW_WF4;
-  r4.x = r->est->d;
-  r4.q = r->Q;
-  r4.post = c1.sf;
-  r4.partials = r->partials;
-  const int nb = row_blocks<T>(r->nx, r->ny);
-  std::vector<Ev> evs;
-  for (int k = 0; k < n; ++k) {
-    if ((rc = record_start(r, evs, 0))) return rc;
-    launch_col<T>(c1, 1, S_(r));
-    if ((rc = record_stop(r, evs))) return rc;
-    if ((rc = record_start(r, evs, 1))) return rc;
-    launch_row<T>(r2, S_(r));
-    if ((rc = record_stop(r, evs))) return rc;
-    if ((rc = record_start(r, evs, 0))) return rc;
-    launch_col<T>(c3, 1, S_(r));
-    if ((rc = record_stop(r, evs))) return rc;
-    if ((rc = record_start(r, evs, 1))) return rc;
-    launch_row<T>(r4, S_(r));
-    if ((rc = record_stop(r, evs))) return rc;
-    k_final_sum2(r->partials, nb, slots + k, nullptr, S_(r));
-    LAUNCHCHK();
-  }
-  r->est->version++;
-  r->w_version = r->est->version;
   r->last_err = slots + (n - 1);
   r->h_err_valid = false;
   return collect_events(r, evs);
@@ -865,6 +956,8 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   r->ring_cap = 4096;
   r->path = NADIA_PATH_FUSED;
   r->beta = 0.9;
+  r->col_v = r->row_v = -1;
+  r->tile_fixed = false;
   const size_t cb = r->npix * r->esz, rb = r->npix * r->rsz;
   int rc = 0;
   if ((rc = alloc(&r->S, rb)) || (rc = alloc(&r->Q, rb)) || (rc = alloc(&r->W, 2 * cb)) ||
@@ -1034,8 +1127,7 @@ int nadia_recon_iterate(nadia_recon* r, int n, double* errors) {
     if (chunk > r->ring_cap) chunk = r->ring_cap;
     if (nb) chunk = 1;  // the n-best bookkeeping needs each error on the host
     if (fused_ok(r)) {
-      if (r->kind == NADIA_FRESNEL_WF) DISPATCH(r->prec, rc = iterate_fused_wf<T>(r, chunk, r->errs));
-      else DISPATCH(r->prec, rc = iterate_fused_pf<T>(r, chunk, r->errs));
+      DISPATCH(r->prec, rc = iterate_fused<T>(r, chunk, r->errs));
       if (rc) return rc;
     } else {
       for (int k = 0; k < chunk; ++k) {
@@ -1441,6 +1533,28 @@ int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size) {
 }
 
 // ---- measurement ---------------------------------------------------------
+
+int nadia_recon_set_tile_variants(nadia_recon* r, int col_variant, int row_variant) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  if (col_variant < 0 || row_variant < 0) {  // back to the autotuner
+    r->tile_fixed = false;
+    r->col_v = r->row_v = -1;
+    return 0;
+  }
+  if (col_variant >= kColVariants || row_variant >= kRowVariants)
+    return fail(NADIA_EINVAL, "tile variant out of range (column < %d, row < %d)", kColVariants, kRowVariants);
+  r->col_v = col_variant;
+  r->row_v = row_variant;
+  r->tile_fixed = true;
+  return 0;
+}
+
+int nadia_recon_get_tile_variants(const nadia_recon* r, int* col_variant, int* row_variant) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  if (col_variant) *col_variant = r->col_v;
+  if (row_variant) *row_variant = r->row_v;
+  return 0;
+}
 
 int nadia_recon_set_timing(nadia_recon* r, int on) {
   if (!r) return fail(NADIA_EINVAL, "null recon");

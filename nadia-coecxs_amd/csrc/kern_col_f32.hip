@@ -1,2 +1,3 @@
 #define REAL float
+#define NADIA_PASS_COL
 #include "kern_pass.inc"

@@ -1,2 +1,3 @@
 #define REAL double
+#define NADIA_PASS_ROW
 #include "kern_pass.inc"

@@ -73,11 +73,12 @@ struct ConvArgs {
   double* partials; // one max per block
 };
 
-template <typename T> void launch_row(const RowArgs& a, hipStream_t st);
-template <typename T> void launch_col(const ColArgs& a, int nfields, hipStream_t st);
-template <typename T> int col_tile(int nx);
-template <typename T> int row_tile(int ny);
-template <typename T> int row_blocks(int nx, int ny);
+// v selects the tile variant (plan.h: col_cfg / row_cfg); all variants of a
+// length compute identical results
+template <typename T> void launch_row(const RowArgs& a, int v, hipStream_t st);
+template <typename T> void launch_col(const ColArgs& a, int nfields, int v, hipStream_t st);
+template <typename T> int col_tile(int nx, int v);
+template <typename T> int row_blocks(int nx, int ny, int v);
 
 // element-wise and reduction kernels (kern_misc.inc)
 template <typename T> void k_invert(void* buf, int nx, int ny, int scale, hipStream_t st);

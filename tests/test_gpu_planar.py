@@ -255,3 +255,26 @@ def test_fp32_against_fp64_oracle(nd, oracle, synth):
     o.iterate()
     assert rel(est.get().astype(np.complex128), xo) <= 1e-5
     assert abs(e1 - o.get_error()) <= 1e-5 * o.get_error()
+
+
+@pytest.mark.parametrize("double", [False, True], ids=["fp32", "fp64"])
+def test_tile_variants_bit_identical(nd, synth, double):
+    """Every tile variant of the fused passes uses the same radix plan, so the
+    results are bit-identical whichever variant the autotuner picks."""
+    d = synth.planar_block(n=512, block=160, support=200)
+    cdt = np.complex128 if double else np.complex64
+    outs = []
+    for cv in range(4):
+        for rv in range(3):
+            est = nd.Complex2D(512, 512, double=double, data=d["x0"].astype(cdt))
+            g = nd.PlanarCDI(est)
+            g.set_support(d["support"])
+            g.set_intensity(d["intensity"])
+            g.set_algorithm(nd.HIO)
+            g.set_tile_variants(cv, rv)
+            e = g.iterate(4)
+            assert g.get_tile_variants() == (cv, rv)
+            outs.append((est.get(), e))
+    for x, e in outs[1:]:
+        assert np.array_equal(x, outs[0][0])
+        assert np.array_equal(e, outs[0][1])
