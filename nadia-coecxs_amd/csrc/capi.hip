@@ -125,6 +125,7 @@ void* twiddles(DeviceCtx* c, int n, int prec, bool row) {
 enum { A_PSF = 0, A_PFS = 1, A_PS = 2, A_PF = 3, A_PI = 4 };
 
 int alloc(void** p, size_t bytes);
+int env_int(const char* name, int dflt);
 
 }  // namespace
 
@@ -212,7 +213,8 @@ int fft2d_T(DeviceCtx* c, void* buf, int nx, int ny, int dir) {
   ca.buf = buf;
   ca.dir = dir;
   ca.scale = 1.0;
-  launch_col<T>(ca, 1, 0, c->stream);
+  static const int fft_cv = env_int("NADIA_FFT_COL_VARIANT", 0);
+  launch_col<T>(ca, 1, fft_cv, c->stream);
   LAUNCHCHK();
   return 0;
 }

@@ -12,6 +12,40 @@ template <> struct Vec2<float> { using type = float2; };
 template <> struct Vec2<double> { using type = double2; };
 template <typename T> using c2_t = typename Vec2<T>::type;
 
+// ---- buffer-resource memory access ----------------------------------------
+// Tile loads and stores go through buffer instructions: one per-thread 32-bit
+// byte offset (VGPR) shared by all P slots of a thread, the slot stride in the
+// scalar offset (SGPR), so the P strided accesses of a column tile cost no
+// per-slot address registers.  Raw buffers with stride 0: num_records is the
+// size in bytes; accesses beyond it read 0 and write nothing.
+typedef unsigned int nadia_u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int nadia_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
+  // 0x00020000: the gfx9-family descriptor word 3 (32-bit data, raw access)
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+template <typename C>
+__device__ __forceinline__ C bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(C) == 8) return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
+
+template <typename C>
+__device__ __forceinline__ void bstore(C v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(C) == 8)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(nadia_u32x2, v), r, voff, soff, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(nadia_u32x4, v), r, voff, soff, 0);
+}
+
+template <typename T>
+__device__ __forceinline__ T bload_real(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {

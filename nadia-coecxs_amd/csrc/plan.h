@@ -85,9 +85,9 @@ constexpr int col_batch(int n, int esz) {
 }
 
 // ---- tile variants chosen at run time by the autotuner --------------------
-// Every variant of a length uses the same radix plan, so all of them produce
-// bit-identical results; they differ in points per thread, transforms per
-// workgroup and the occupancy target given to the register allocator.
+// Every variant of a length keeps the radix plan and the points per thread,
+// so all of them produce bit-identical results; they differ in transforms per
+// workgroup and in the occupancy target given to the register allocator.
 struct TileCfg {
   int p;        // points per thread
   int batch;    // transforms (columns or rows) per workgroup
@@ -95,8 +95,8 @@ struct TileCfg {
   int minw;     // waves per SIMD the register allocator must allow
 };
 
-constexpr int kColVariants = 4;
-constexpr int kRowVariants = 3;
+constexpr int kColVariants = 3;
+constexpr int kRowVariants = 2;
 
 // lengths that get more than the default variant
 constexpr bool tuned_size(int n) { return n == 512 || n == 1024 || n == 2048 || n == 4096; }
@@ -104,14 +104,7 @@ constexpr bool tuned_size(int n) { return n == 512 || n == 1024 || n == 2048 || 
 constexpr TileCfg col_cfg(int n, int esz, int v) {
   TileCfg c{col_points(n), col_batch(n, esz), 0, 1};
   if (!tuned_size(n)) v = 0;
-  if (v == 2) {  // 16 points per thread, up to 64-byte segments
-    c.p = 16;
-    int b = 64 / esz;
-    while (b > 1 && (n * b * esz > 128 * 1024 || b * (n / 16) > 1024)) b /= 2;
-    c.batch = b;
-  } else if (v == 3) {  // half the columns: smaller workgroups, more per CU
-    c.batch = c.batch > 1 ? c.batch / 2 : 1;
-  }
+  if (v == 2) c.batch = c.batch > 1 ? c.batch / 2 : 1;  // smaller workgroups, more per CU
   c.threads = c.batch * (n / c.p);
   if (v == 1) {  // ask for registers that let two workgroups share a CU
     const int w = c.threads / 128;
@@ -123,13 +116,7 @@ constexpr TileCfg col_cfg(int n, int esz, int v) {
 constexpr TileCfg row_cfg(int n, int v) {
   TileCfg c{row_points(n), row_batch(n), 0, 1};
   if (!tuned_size(n)) v = 0;
-  if (v == 1) {  // 16 points per thread
-    c.p = 16;
-    const int b = 256 / (n / 16);
-    c.batch = b < 1 ? 1 : b;
-  } else if (v == 2) {  // two rows' worth of threads per workgroup
-    c.batch = c.batch * 2;
-  }
+  if (v == 1) c.batch = c.batch * 2;  // two rows' worth of threads per workgroup
   c.threads = c.batch * (n / c.p);
   return c;
 }
