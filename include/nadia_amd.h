@@ -62,6 +62,7 @@ extern "C" {
 
 typedef struct nadia_field nadia_field; /* device-resident complex field */
 typedef struct nadia_recon nadia_recon; /* one reconstruction (BaseCDI object) */
+typedef struct nadia_batch nadia_batch; /* reconstructions iterated together */
 
 const char* nadia_last_error(void);
 int nadia_version(void);
@@ -181,6 +182,24 @@ int nadia_wf_setup(nadia_recon* r, double wavelength, double zone_focal_length,
                    double focal_detector_length, double pixel_size);
 /* set_support(z_diameter, size) (src/FresnelCDI_WF.c++:177-196) */
 int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size);
+
+/* ---- batches (no counterpart in the reference) ----
+ * A batch iterates many reconstructions together: those of the same
+ * precision, kind and size share every launch of the fused passes, so small
+ * jobs fill the GPU and the phases of different jobs overlap.  Each job keeps
+ * its own state, algorithm and error ring; its results are bit-identical to
+ * iterating it alone (same kernels, same per-job reductions).  The
+ * reconstructions stay owned by the caller; all must live on the batch's
+ * device. */
+int nadia_batch_create(nadia_batch** out);
+int nadia_batch_destroy(nadia_batch* b);
+int nadia_batch_add(nadia_batch* b, nadia_recon* r);
+int nadia_batch_remove(nadia_batch* b, nadia_recon* r);
+int nadia_batch_size(const nadia_batch* b, int* n);
+/* n iterations of every job (each job's iterate()); errors (may be NULL)
+ * receives n x size values, errors[k*size + j] = get_error() of job j (in the
+ * order added) after iteration k. */
+int nadia_batch_iterate(nadia_batch* b, int n, double* errors);
 
 /* ---- tuning and measurement ---- */
 

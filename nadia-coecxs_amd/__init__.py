@@ -99,6 +99,12 @@ def _sig(lib):
         "nadia_fresnel_set_transmission_function": (I, [P, P, P]),
         "nadia_wf_setup": (I, [P, D, D, D, D]),
         "nadia_wf_set_support": (I, [P, D, D]),
+        "nadia_batch_create": (I, [PP]),
+        "nadia_batch_destroy": (I, [P]),
+        "nadia_batch_add": (I, [P, P]),
+        "nadia_batch_remove": (I, [P, P]),
+        "nadia_batch_size": (I, [P, IP]),
+        "nadia_batch_iterate": (I, [P, I, DP]),
         "nadia_recon_set_tile_variants": (I, [P, I, I]),
         "nadia_recon_get_tile_variants": (I, [P, IP, IP]),
         "nadia_recon_set_timing": (I, [P, I]),
@@ -421,3 +427,44 @@ class FresnelCDI_WF(BaseCDI):
 
     def set_support(self, z_diameter, size=1.01):  # hides BaseCDI::set_support like the reference
         _check(library().nadia_wf_set_support(self.h, float(z_diameter), float(size)))
+
+
+class Batch:
+    """Reconstructions iterated together (nadia_batch_*): jobs of the same
+    precision, kind and size share every launch of the fused passes.  Each
+    job keeps its own state and gives the same results as alone."""
+
+    def __init__(self, jobs=()):
+        h = ctypes.c_void_p()
+        _check(library().nadia_batch_create(ctypes.byref(h)))
+        self.h = h
+        self.jobs = []
+        for j in jobs:
+            self.add(j)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.nadia_batch_destroy(self.h)
+            self.h = None
+
+    def add(self, job):
+        _check(library().nadia_batch_add(self.h, job.h))
+        self.jobs.append(job)
+
+    def remove(self, job):
+        _check(library().nadia_batch_remove(self.h, job.h))
+        self.jobs.remove(job)
+
+    def __len__(self):
+        return len(self.jobs)
+
+    def iterate(self, n=1):
+        """n iterations of every job; returns errors with shape (n, len(self))."""
+        errs = np.zeros((int(n), len(self.jobs)))
+        if self.jobs:
+            _check(library().nadia_batch_iterate(self.h, int(n), errs.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return errs
+
+    def iterate_async(self, n=1):
+        _check(library().nadia_batch_iterate(self.h, int(n), None))
+

@@ -182,6 +182,17 @@ struct nadia_recon {
   double pass_ms[2];
   long long pass_n[2];
   std::vector<hipEvent_t> ev_pool;
+  // job descriptors of the fused passes (device): [0] this reconstruction,
+  // [1] scratch copy used by the autotuner
+  JobDesc* djob;
+  double* tune_errs;
+};
+
+struct nadia_batch {
+  int dev;
+  std::vector<nadia_recon*> jobs;
+  JobDesc* djobs;  // device descriptors, grouped
+  int cap;
 };
 
 namespace {
@@ -203,7 +214,7 @@ int fft2d_T(DeviceCtx* c, void* buf, int nx, int ny, int dir) {
   ra.dst = buf;
   ra.dir = dir;
   ra.scale = 1.0;
-  launch_row<T>(ra, 0, c->stream);
+  launch_row<T>(ra, 1, 0, c->stream);
   LAUNCHCHK();
   ColArgs ca{};
   ca.mode = COL_FFT;
@@ -441,7 +452,7 @@ int fused_prepare(nadia_recon* r, bool two) {
     } else {
       ra.dir = -1;
     }
-    launch_row<T>(ra, rv, S_(r));
+    launch_row<T>(ra, 1, rv, S_(r));
     LAUNCHCHK();
     r->w_valid = true;
     r->w_version = r->est->version;
@@ -459,7 +470,7 @@ int fused_prepare(nadia_recon* r, bool two) {
     ra.dst = W2;
     ra.dir = -1;
     ra.scale = 1.0;
-    launch_row<T>(ra, rv, S_(r));
+    launch_row<T>(ra, 1, rv, S_(r));
     LAUNCHCHK();
     r->w2_valid = true;
     r->w2_version = r->est->version;
@@ -472,98 +483,108 @@ bool two_fields(const nadia_recon* r) {
   return r->kind != NADIA_FRESNEL_WF && r->algorithm != NADIA_ER && r->alg[A_PFS] != 0;
 }
 
-// Queue one fused iteration on the buffers X (estimate) and W (work fields),
-// with error slot `slot`, column/row tile variants cv/rv.  PlanarCDI and
-// FresnelCDI: column pass (modulus projection) + row pass (update).
-// FresnelCDI_WF: four passes and a final error reduction.
+// Host copy of the reconstruction's job descriptor for the fused passes, with
+// the estimate X and work fields W given explicitly (the autotuner points
+// them at scratch copies).
+JobDesc make_jobdesc(const nadia_recon* r, void* X, void* W, double* errs) {
+  JobDesc j{};
+  j.x = X;
+  j.w = W;
+  j.s = r->S;
+  j.q = r->Q;
+  j.ill = r->L;
+  j.partials = r->partials;
+  j.errs = errs;
+  for (int i = 0; i < 5; i++) j.coef[i] = r->alg[i];
+  j.er = r->algorithm == NADIA_ER;
+  // get_error() reports the last projection the reference evaluates: P_F x
+  // when a PF or PSF term exists, else P_F P_S x
+  j.err_field = (r->algorithm == NADIA_ER || r->alg[A_PF] != 0 || r->alg[A_PSF] != 0) ? 0 : 1;
+  j.phase_factor = r->phase_factor;
+  j.wf_sx = r->wf_sx;
+  j.wf_sy = r->wf_sy;
+  j.wf_z12 = r->z12;
+  j.wf_z23 = r->z23;
+  j.wf_k = r->wf_k;
+  j.wf_norm = r->wf_norm;
+  return j;
+}
+
+// Reconstructions whose fused passes can share launches.
+struct GroupKey {
+  int prec, kind, nx, ny, two;
+  bool operator<(const GroupKey& o) const {
+    if (prec != o.prec) return prec < o.prec;
+    if (kind != o.kind) return kind < o.kind;
+    if (nx != o.nx) return nx < o.nx;
+    if (ny != o.ny) return ny < o.ny;
+    return two < o.two;
+  }
+};
+
+GroupKey group_key(const nadia_recon* r) { return GroupKey{r->prec, r->kind, r->nx, r->ny, two_fields(r) ? 1 : 0}; }
+
+// Queue one fused iteration of `njobs` reconstructions sharing key g, their
+// descriptors in djobs (device); error ring slot `iter`; column / row tile
+// variants cv / rv.  PlanarCDI and FresnelCDI: column pass (modulus
+// projection) + row pass (update).  FresnelCDI_WF: four passes and a final
+// error reduction.  evs (may be null) receives per-pass timing events.
 template <typename T>
-int fused_iteration(nadia_recon* r, void* X, void* W, double* slot, int cv, int rv, std::vector<Ev>* evs) {
-  void* twy = twiddles(r->ctx, r->ny, r->prec, true);
-  void* twx = twiddles(r->ctx, r->nx, r->prec, false);
+int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njobs, int iter, int cv, int rv,
+                 nadia_recon* timer, std::vector<Ev>* evs) {
+  void* twy = twiddles(c, g.ny, g.prec, true);
+  void* twx = twiddles(c, g.nx, g.prec, false);
   if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
-  const double sf = 1.0 / std::sqrt((double)r->nx * (double)r->ny);
+  hipStream_t st = c->stream;
   ColArgs ca{};
-  ca.nx = r->nx;
-  ca.ny = r->ny;
+  ca.jobs = djobs;
+  ca.nfields = g.two ? 2 : 1;
+  ca.nx = g.nx;
+  ca.ny = g.ny;
   ca.tw = twx;
-  ca.buf = W;
-  ca.field_stride = r->npix;
-  ca.q = r->Q;
-  ca.ill = r->L;
-  ca.sf = sf;
-  ca.phase_factor = r->phase_factor;
-  ca.xmid = (r->nx - 1) / 2.0;
-  ca.ymid = (r->ny - 1) / 2.0;
-  ca.partials = r->partials;
+  ca.sf = 1.0 / std::sqrt((double)g.nx * (double)g.ny);
+  ca.xmid = (g.nx - 1) / 2.0;
+  ca.ymid = (g.ny - 1) / 2.0;
   RowArgs ra{};
-  ra.nrows = r->nx;
-  ra.ny = r->ny;
+  ra.jobs = djobs;
+  ra.iter = iter;
+  ra.nrows = g.nx;
+  ra.ny = g.ny;
   ra.tw = twy;
-  ra.src = W;
-  ra.dst = W;
-  ra.x = X;
-  ra.s = r->S;
-  ra.post = sf;
+  ra.two = g.two;
+  ra.post = ca.sf;
   int rc;
-  if (r->kind != NADIA_FRESNEL_WF) {
-    const bool two = two_fields(r);
-    const int G = r->ny / col_tile<T>(r->nx, cv);
-    // get_error() reports the last projection the reference evaluates: P_F x
-    // when a PF or PSF term exists, else P_F P_S x
-    const int err_field = (r->algorithm == NADIA_ER || r->alg[A_PF] != 0 || r->alg[A_PSF] != 0) ? 0 : 1;
-    ca.mode = r->kind == NADIA_FRESNEL ? COL_FRESNEL : COL_PLANAR;
-    ra.mode = ROW_UPDATE;
-    void* W2 = (char*)W + r->npix * r->esz;
-    ra.src2 = two ? W2 : nullptr;
-    ra.dst2 = two ? W2 : nullptr;
-    ra.er = r->algorithm == NADIA_ER;
-    ra.a = r->alg[A_PSF];
-    ra.b = r->alg[A_PFS];
-    ra.c = r->alg[A_PS];
-    ra.d = r->alg[A_PF];
-    ra.e = r->alg[A_PI];
-    ra.fin_partials = r->partials + 2LL * G * err_field;
-    ra.fin_n = G;
-    ra.err_out = slot;
-    if (evs && (rc = record_start(r, *evs, 0))) return rc;
-    launch_col<T>(ca, two ? 2 : 1, cv, S_(r));
-    if (evs && (rc = record_stop(r, *evs))) return rc;
-    LAUNCHCHK();
-    if (evs && (rc = record_start(r, *evs, 1))) return rc;
-    launch_row<T>(ra, rv, S_(r));
-    if (evs && (rc = record_stop(r, *evs))) return rc;
+  auto col = [&](int mode) -> int {
+    ca.mode = mode;
+    if (evs && (rc = record_start(timer, *evs, 0))) return rc;
+    launch_col<T>(ca, njobs, cv, st);
+    if (evs && (rc = record_stop(timer, *evs))) return rc;
     LAUNCHCHK();
     return 0;
+  };
+  auto row = [&](int mode) -> int {
+    ra.mode = mode;
+    if (evs && (rc = record_start(timer, *evs, 1))) return rc;
+    launch_row<T>(ra, njobs, rv, st);
+    if (evs && (rc = record_stop(timer, *evs))) return rc;
+    LAUNCHCHK();
+    return 0;
+  };
+  if (g.kind != NADIA_FRESNEL_WF) {
+    ra.fin_n = g.ny / col_tile<T>(g.nx, cv);
+    if ((rc = col(g.kind == NADIA_FRESNEL ? COL_FRESNEL : COL_PLANAR))) return rc;
+    return row(ROW_UPDATE);
   }
-  ca.wf_sx = r->wf_sx;
-  ca.wf_sy = r->wf_sy;
-  ca.wf_z12 = r->z12;
-  ca.wf_z23 = r->z23;
-  ca.wf_k = r->wf_k;
-  ca.wf_norm = r->wf_norm;
-  ColArgs c3 = ca;
-  ca.mode = COL_WF1;
-  c3.mode = COL_WF3;
-  ra.mode = ROW_WF2;
-  RowArgs r4 = ra;
-  r4.mode = ROW_WF4;
-  r4.q = r->Q;
-  r4.partials = r->partials;
-  const int nb = row_blocks<T>(r->nx, r->ny, rv);
-  if (evs && (rc = record_start(r, *evs, 0))) return rc;
-  launch_col<T>(ca, 1, cv, S_(r));
-  if (evs && (rc = record_stop(r, *evs))) return rc;
-  if (evs && (rc = record_start(r, *evs, 1))) return rc;
-  launch_row<T>(ra, rv, S_(r));
-  if (evs && (rc = record_stop(r, *evs))) return rc;
-  if (evs && (rc = record_start(r, *evs, 0))) return rc;
-  launch_col<T>(c3, 1, cv, S_(r));
-  if (evs && (rc = record_stop(r, *evs))) return rc;
-  if (evs && (rc = record_start(r, *evs, 1))) return rc;
-  launch_row<T>(r4, rv, S_(r));
-  if (evs && (rc = record_stop(r, *evs))) return rc;
-  k_final_sum2(r->partials, nb, slot, nullptr, S_(r));
+  if ((rc = col(COL_WF1)) || (rc = row(ROW_WF2)) || (rc = col(COL_WF3)) || (rc = row(ROW_WF4))) return rc;
+  k_final_jobs(djobs, njobs, row_blocks<T>(g.nx, g.ny, rv), iter, st);
   LAUNCHCHK();
+  return 0;
+}
+
+int upload_jobs(hipStream_t st, JobDesc* dst, const JobDesc* src, int n) {
+  HIPCHK(hipMemcpyAsync(dst, src, sizeof(JobDesc) * n, hipMemcpyHostToDevice, st));
+  // the host array may be reused right after: wait for the copy
+  HIPCHK(hipStreamSynchronize(st));
   return 0;
 }
 
@@ -574,17 +595,7 @@ int fused_iteration(nadia_recon* r, void* X, void* W, double* slot, int cv, int 
 // state is not touched).  Choices are cached for the life of the process.
 // NADIA_TUNE=0 disables it (default variants); NADIA_COL_VARIANT /
 // NADIA_ROW_VARIANT force a choice.
-struct TuneKey {
-  int prec, kind, nx, ny, two;
-  bool operator<(const TuneKey& o) const {
-    if (prec != o.prec) return prec < o.prec;
-    if (kind != o.kind) return kind < o.kind;
-    if (nx != o.nx) return nx < o.nx;
-    if (ny != o.ny) return ny < o.ny;
-    return two < o.two;
-  }
-};
-std::map<TuneKey, std::pair<int, int>> g_tuned;
+std::map<GroupKey, std::pair<int, int>> g_tuned;
 
 int env_int(const char* name, int dflt) {
   const char* v = getenv(name);
@@ -592,13 +603,14 @@ int env_int(const char* name, int dflt) {
 }
 
 template <typename T>
-int time_variant(nadia_recon* r, void* X, void* W, int cv, int rv, int reps, float* ms) {
+int time_variant(nadia_recon* r, const JobDesc* dj, int cv, int rv, int reps, float* ms) {
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
   int rc = 0;
+  const GroupKey g = group_key(r);
   HIPCHK(hipEventRecord(a, S_(r)));
-  for (int k = 0; k < reps && !rc; ++k) rc = fused_iteration<T>(r, X, W, r->scal + 12, cv, rv, nullptr);
+  for (int k = 0; k < reps && !rc; ++k) rc = launch_fused<T>(r->ctx, g, dj, 1, k, cv, rv, nullptr, nullptr);
   HIPCHK(hipEventRecord(b, S_(r)));
   HIPCHK(hipEventSynchronize(b));
   HIPCHK(hipEventElapsedTime(ms, a, b));
@@ -606,6 +618,8 @@ int time_variant(nadia_recon* r, void* X, void* W, int cv, int rv, int reps, flo
   (void)hipEventDestroy(b);
   return rc;
 }
+
+constexpr int kTuneReps = 16;
 
 template <typename T>
 int autotune(nadia_recon* r) {
@@ -616,7 +630,7 @@ int autotune(nadia_recon* r) {
     r->row_v = (fr >= 0 && fr < kRowVariants) ? fr : 0;
     return 0;
   }
-  const TuneKey key{r->prec, r->kind, r->nx, r->ny, two_fields(r) ? 1 : 0};
+  const GroupKey key = group_key(r);
   {
     std::lock_guard<std::mutex> lk(g_mu);
     auto it = g_tuned.find(key);
@@ -635,12 +649,23 @@ int autotune(nadia_recon* r) {
   }
   HIPCHK(hipMemcpyAsync(X, r->est->d, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
   HIPCHK(hipMemcpyAsync(W, r->W, 2 * r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
-  const int reps = 6;
+  const JobDesc hj = make_jobdesc(r, X, W, r->tune_errs);
+  if ((rc = upload_jobs(S_(r), r->djob + 1, &hj, 1))) return rc;
+  // each candidate: a warm-up, then the best of two timed runs
+  auto measure = [&](int cv, int rv, float* best) -> int {
+    float ms = 0;
+    *best = 1e30f;
+    if ((rc = time_variant<T>(r, r->djob + 1, cv, rv, 2, &ms))) return rc;
+    for (int k = 0; k < 2; ++k) {
+      if ((rc = time_variant<T>(r, r->djob + 1, cv, rv, kTuneReps, &ms))) return rc;
+      if (ms < *best) *best = ms;
+    }
+    return 0;
+  };
   int best_c = 0, best_r = 0;
   float best = 1e30f, ms = 0;
   for (int cv = 0; cv < kColVariants && !rc; ++cv) {
-    if ((rc = time_variant<T>(r, X, W, cv, 0, 2, &ms))) break;  // warm-up
-    if ((rc = time_variant<T>(r, X, W, cv, 0, reps, &ms))) break;
+    if ((rc = measure(cv, 0, &ms))) break;
     if (ms < best) {
       best = ms;
       best_c = cv;
@@ -648,8 +673,7 @@ int autotune(nadia_recon* r) {
   }
   best = 1e30f;
   for (int rv = 0; rv < kRowVariants && !rc; ++rv) {
-    if ((rc = time_variant<T>(r, X, W, best_c, rv, 2, &ms))) break;
-    if ((rc = time_variant<T>(r, X, W, best_c, rv, reps, &ms))) break;
+    if ((rc = measure(best_c, rv, &ms))) break;
     if (ms < best) {
       best = ms;
       best_r = rv;
@@ -666,26 +690,38 @@ int autotune(nadia_recon* r) {
   return 0;
 }
 
+// Make W current and the tile variants chosen; upload the descriptor.
 template <typename T>
-int iterate_fused(nadia_recon* r, int n, double* slots) {
+int fused_ready(nadia_recon* r) {
   int rc;
-  if (r->col_v < 0 || r->row_v < 0 || !r->tile_fixed) {
-    if ((rc = fused_prepare<T>(r, two_fields(r)))) return rc;
-    if ((rc = autotune<T>(r))) return rc;
-  }
   if ((rc = fused_prepare<T>(r, two_fields(r)))) return rc;
-  std::vector<Ev> evs;
-  for (int k = 0; k < n; ++k)
-    if ((rc = fused_iteration<T>(r, r->est->d, r->W, slots + k, r->col_v, r->row_v, r->timing ? &evs : nullptr)))
-      return rc;
+  if ((rc = autotune<T>(r))) return rc;
+  const JobDesc hj = make_jobdesc(r, r->est->d, r->W, r->errs);
+  return upload_jobs(S_(r), r->djob, &hj, 1);
+}
+
+// After fused iterations: X changed, W is the row transform of the new X.
+void fused_done(nadia_recon* r, int n) {
   r->est->version++;
   r->w_version = r->est->version;
   if (two_fields(r)) {
     r->w2_version = r->est->version;
     r->w2_sversion = r->s_version;
   }
-  r->last_err = slots + (n - 1);
+  r->last_err = r->errs + (n - 1);
   r->h_err_valid = false;
+}
+
+template <typename T>
+int iterate_fused(nadia_recon* r, int n) {
+  int rc;
+  if ((rc = fused_ready<T>(r))) return rc;
+  std::vector<Ev> evs;
+  const GroupKey g = group_key(r);
+  for (int k = 0; k < n; ++k)
+    if ((rc = launch_fused<T>(r->ctx, g, r->djob, 1, k, r->col_v, r->row_v, r, r->timing ? &evs : nullptr)))
+      return rc;
+  fused_done(r, n);
   return collect_events(r, evs);
 }
 
@@ -966,7 +1002,8 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
       (rc = alloc(&r->conv, rb)) || (rc = alloc(&r->rtmp, rb)) || (rc = alloc(&r->gwin, 64 * 8)) ||
       (rc = alloc((void**)&r->partials, sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16))) ||
       (rc = alloc((void**)&r->errs, sizeof(double) * (r->ring_cap + 8))) ||
-      (rc = alloc((void**)&r->scal, sizeof(double) * 16))) {
+      (rc = alloc((void**)&r->scal, sizeof(double) * 16)) || (rc = alloc((void**)&r->djob, sizeof(JobDesc) * 2)) ||
+      (rc = alloc((void**)&r->tune_errs, sizeof(double) * 64))) {
     delete r;
     return rc;
   }
@@ -994,7 +1031,7 @@ int nadia_recon_destroy(nadia_recon* r) {
   if (!r) return 0;
   (void)hipStreamSynchronize(S_(r));
   void* ptrs[] = {r->S, r->Q, r->beam, r->W, r->t[0], r->t[1], r->t[2], r->t[3], r->conv, r->rtmp,
-                  r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls};
+                  r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (void* p : r->best) (void)hipFree(p);
@@ -1129,7 +1166,7 @@ int nadia_recon_iterate(nadia_recon* r, int n, double* errors) {
     if (chunk > r->ring_cap) chunk = r->ring_cap;
     if (nb) chunk = 1;  // the n-best bookkeeping needs each error on the host
     if (fused_ok(r)) {
-      DISPATCH(r->prec, rc = iterate_fused<T>(r, chunk, r->errs));
+      DISPATCH(r->prec, rc = iterate_fused<T>(r, chunk));
       if (rc) return rc;
     } else {
       for (int k = 0; k < chunk; ++k) {
@@ -1535,6 +1572,138 @@ int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size) {
 }
 
 // ---- measurement ---------------------------------------------------------
+
+// ---- batches ------------------------------------------------------------
+
+int nadia_batch_create(nadia_batch** out) {
+  if (!out) return fail(NADIA_EINVAL, "null output pointer");
+  nadia_batch* b = new nadia_batch{};
+  b->dev = current_device();
+  *out = b;
+  return 0;
+}
+
+int nadia_batch_destroy(nadia_batch* b) {
+  if (!b) return 0;
+  DeviceCtx* c = dev_ctx(b->dev);
+  if (c) (void)hipStreamSynchronize(c->stream);
+  if (b->djobs) (void)hipFree(b->djobs);
+  delete b;
+  return 0;
+}
+
+int nadia_batch_add(nadia_batch* b, nadia_recon* r) {
+  if (!b || !r) return fail(NADIA_EINVAL, "null argument");
+  if (r->dev != b->dev) return fail(NADIA_EINVAL, "reconstruction lives on another device than the batch");
+  for (nadia_recon* j : b->jobs)
+    if (j == r) return fail(NADIA_EINVAL, "reconstruction already in the batch");
+  b->jobs.push_back(r);
+  return 0;
+}
+
+int nadia_batch_remove(nadia_batch* b, nadia_recon* r) {
+  if (!b || !r) return fail(NADIA_EINVAL, "null argument");
+  for (size_t i = 0; i < b->jobs.size(); i++)
+    if (b->jobs[i] == r) {
+      b->jobs.erase(b->jobs.begin() + i);
+      return 0;
+    }
+  return fail(NADIA_EINVAL, "reconstruction not in the batch");
+}
+
+int nadia_batch_size(const nadia_batch* b, int* n) {
+  if (!b || !n) return fail(NADIA_EINVAL, "null argument");
+  *n = (int)b->jobs.size();
+  return 0;
+}
+
+int nadia_batch_iterate(nadia_batch* b, int n, double* errors) {
+  if (!b) return fail(NADIA_EINVAL, "null batch");
+  const int nj = (int)b->jobs.size();
+  if (n <= 0 || nj == 0) return 0;
+  DeviceCtx* c = dev_ctx(b->dev);
+  if (!c) return fail(NADIA_EHIP, "stream creation failed");
+  int rc;
+  bool per_iteration = false;
+  for (nadia_recon* r : b->jobs) {
+    if (!r->has_intensity) return fail(NADIA_ESTATE, "a reconstruction in the batch has no intensity");
+    if ((rc = need_setup(r))) return rc;
+    if (r->n_best > 0 && r->kind != NADIA_FRESNEL_WF) per_iteration = true;
+  }
+  if (b->cap < nj) {
+    if (b->djobs) (void)hipFree(b->djobs);
+    b->djobs = nullptr;
+    if ((rc = alloc((void**)&b->djobs, sizeof(JobDesc) * nj))) return rc;
+    b->cap = nj;
+  }
+  int done = 0;
+  while (done < n) {
+    int chunk = n - done;
+    if (chunk > 4096) chunk = 4096;
+    if (per_iteration) chunk = 1;
+    // group the fused-path jobs; the rest run the unfused path one by one
+    std::map<GroupKey, std::vector<nadia_recon*>> groups;
+    std::vector<nadia_recon*> plain;
+    for (nadia_recon* r : b->jobs) {
+      if (fused_ok(r)) groups[group_key(r)].push_back(r);
+      else plain.push_back(r);
+    }
+    struct Launch {
+      GroupKey g;
+      JobDesc* d;
+      int count, cv, rv;
+    };
+    std::vector<Launch> launches;
+    std::vector<JobDesc> host;
+    host.reserve(nj);
+    for (auto& kv : groups) {
+      for (nadia_recon* r : kv.second) {
+        DISPATCH(r->prec, rc = fused_ready<T>(r));
+        if (rc) return rc;
+      }
+      Launch L{kv.first, b->djobs + host.size(), (int)kv.second.size(), kv.second[0]->col_v, kv.second[0]->row_v};
+      for (nadia_recon* r : kv.second) host.push_back(make_jobdesc(r, r->est->d, r->W, r->errs));
+      launches.push_back(L);
+    }
+    if (!host.empty() && (rc = upload_jobs(c->stream, b->djobs, host.data(), (int)host.size()))) return rc;
+    for (int k = 0; k < chunk; ++k) {
+      for (const Launch& L : launches) {
+        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, L.count, k, L.cv, L.rv, nullptr, nullptr));
+        if (rc) return rc;
+      }
+      for (nadia_recon* r : plain)
+        if ((rc = iterate_plain(r, r->errs + k))) return rc;
+    }
+    for (auto& kv : groups)
+      for (nadia_recon* r : kv.second) fused_done(r, chunk);
+    for (nadia_recon* r : plain) r->est->version++;
+    for (nadia_recon* r : b->jobs)
+      if (per_iteration && r->n_best > 0 && r->kind != NADIA_FRESNEL_WF && (rc = update_n_best(r))) return rc;
+    if (errors) {
+      std::vector<double> tmp(chunk);
+      for (int j = 0; j < nj; ++j) {
+        nadia_recon* r = b->jobs[j];
+        if (has_projection(r)) {
+          HIPCHK(hipMemcpyAsync(tmp.data(), r->errs, chunk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+          HIPCHK(hipStreamSynchronize(c->stream));
+        } else {
+          double v = 0;
+          if ((rc = fetch_error(r, &v))) return rc;
+          for (int k = 0; k < chunk; ++k) tmp[k] = v;
+        }
+        for (int k = 0; k < chunk; ++k) errors[(size_t)(done + k) * nj + j] = tmp[k];
+      }
+    }
+    done += chunk;
+    if (done < n) HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  for (nadia_recon* r : b->jobs)
+    if (r->last_err && r->last_err != r->scal + 8) {
+      HIPCHK(hipMemcpyAsync(r->scal + 8, r->last_err, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      r->last_err = r->scal + 8;
+    }
+  return 0;
+}
 
 int nadia_recon_set_tile_variants(nadia_recon* r, int col_variant, int row_variant) {
   if (!r) return fail(NADIA_EINVAL, "null recon");

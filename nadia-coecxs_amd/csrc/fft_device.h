@@ -48,8 +48,13 @@ __device__ __forceinline__ T bload_real(__amdgpu_buffer_rsrc_t r, unsigned voff,
 
 template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
 template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+// complex product with two explicit fused multiply-adds (the kernels are
+// built without implicit contraction, so every instantiation rounds alike)
 template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
-  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+  return C{__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x)};
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return float2{__builtin_fmaf(a.x, b.x, -(a.y * b.y)), __builtin_fmaf(a.x, b.y, a.y * b.x)};
 }
 // a * (DIR * i)
 template <int DIR, typename C> __device__ __forceinline__ C cmul_i(C a) {

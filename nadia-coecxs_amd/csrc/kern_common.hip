@@ -38,6 +38,23 @@ __global__ void __launch_bounds__(256) final_sum1_kernel(const double* p, int n,
   if (threadIdx.x == 0) out[0] = a;
 }
 
+// one workgroup per job: its row-pass partial sums -> errs[iter]
+__global__ void __launch_bounds__(256) final_jobs_kernel(const JobDesc* jobs, int n, int iter) {
+  __shared__ double red[16];
+  const JobDesc* j = jobs + blockIdx.x;
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    a += j->partials[2 * i];
+    b += j->partials[2 * i + 1];
+  }
+  block_sum2(a, b, red);
+  if (threadIdx.x == 0) j->errs[iter] = a / b;
+}
+
+void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st) {
+  hipLaunchKernelGGL(final_jobs_kernel, dim3(njobs), dim3(256), 0, st, jobs, n, iter);
+}
+
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st) {
   hipLaunchKernelGGL(final_sum2_kernel, dim3(1), dim3(256), 0, st, partials, n, out_ratio, out_pair);
 }

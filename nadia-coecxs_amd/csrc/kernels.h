@@ -21,47 +21,55 @@ enum ColMode {
   COL_WF3 = 4,      // white field: forward column DFT -> x conj(coef) -> forward column DFT
 };
 
+// Per-reconstruction data of the fused passes.  A launch covers one or more
+// reconstructions of the same precision, kind and size; blockIdx.y selects
+// the job (the column pass: job * nfields + field).  Lives in device memory.
+struct JobDesc {
+  void* x;            // estimate
+  void* w;            // work field(s): nfields consecutive nx*ny fields
+  const void* s;      // support
+  const void* q;      // sqrt(I) with the beam-stop sign
+  const void* ill;    // Fresnel white field (detector plane)
+  double* partials;   // this job's partial error sums
+  double* errs;       // this job's error ring (slot = iteration index)
+  double coef[5];     // PSF, PFS, PS, PF, PI
+  int er;             // error-reduction special case
+  int err_field;      // field whose projection error get_error() reports
+  double phase_factor;                                   // Fresnel
+  double wf_sx, wf_sy, wf_z12, wf_z23, wf_k, wf_norm;    // white field
+};
+
 struct RowArgs {
   int mode;
-  int nrows, ny;
-  const void* tw;  // twiddle table of length ny (sign -1 values)
-  // ROW_FFT
+  const JobDesc* jobs;  // fused modes: per-job data (blockIdx.y = job)
+  int iter;             // error ring slot written by this launch
+  int nrows, ny;        // rows per job, row length
+  const void* tw;       // twiddle table of length ny (sign -1 values)
+  // ROW_FFT (one field, no job table)
   const void* src;
   void* dst;
   int dir;
   int pre_sign;  // multiply by (-1)^(x+y) before the transform
   double scale;
-  // ROW_UPDATE (x, support; W2 carries the P_F P_S x field when b != 0)
-  void* x;
-  const void* s;
-  const void* src2;
-  void* dst2;
-  double a, b, c, d, e;  // PSF, PFS, PS, PF, PI coefficients
-  int er;
-  double post;  // factor applied after the inverse row transform
-  // error finalisation of the preceding column pass (block 0)
-  const double* fin_partials;
-  int fin_n;
-  double* err_out;
-  // ROW_WF4: modulus inputs and partial sums of this pass
-  const void* q;
-  double* partials;
+  // ROW_UPDATE
+  int two;       // the jobs carry a second field (P_F P_S x) after the first
+  double post;   // factor applied after the inverse row transform
+  int fin_n;     // number of column-pass blocks per field (error finalisation)
 };
 
 struct ColArgs {
   int mode;
+  const JobDesc* jobs;  // fused modes: per-job data (blockIdx.y = job * nfields + field)
+  int nfields;
   int nx, ny;
   const void* tw;  // twiddle table of length nx
+  // COL_FFT (one field, no job table)
   void* buf;
-  long long field_stride;  // elements between fields (blockIdx.y selects the field)
   int dir;
   double scale;
-  const void* q;    // sqrt(I) with the beam-stop mask in the sign bit
-  const void* ill;  // Fresnel white field in the detector plane
-  double sf;        // 1/sqrt(nx*ny)
-  double phase_factor, xmid, ymid;      // Fresnel quadratic phase
-  double wf_sx, wf_sy, wf_z12, wf_z23, wf_k, wf_norm;  // white-field phase and 1/sqrt(N)
-  double* partials;  // 2 doubles per block per field
+  // fused modes
+  double sf;  // 1/sqrt(nx*ny)
+  double xmid, ymid;
 };
 
 struct ConvArgs {
@@ -75,8 +83,8 @@ struct ConvArgs {
 
 // v selects the tile variant (plan.h: col_cfg / row_cfg); all variants of a
 // length compute identical results
-template <typename T> void launch_row(const RowArgs& a, int v, hipStream_t st);
-template <typename T> void launch_col(const ColArgs& a, int nfields, int v, hipStream_t st);
+template <typename T> void launch_row(const RowArgs& a, int njobs, int v, hipStream_t st);
+template <typename T> void launch_col(const ColArgs& a, int njobs, int v, hipStream_t st);
 template <typename T> int col_tile(int nx, int v);
 template <typename T> int row_blocks(int nx, int ny, int v);
 
@@ -105,6 +113,8 @@ template <typename T> void k_threshold(const void* conv, void* out, int nx, int 
 template <typename T> void k_normalise(const void* src, void* dst, const double* maxv, long long n, hipStream_t st);
 template <typename T> void k_wf_disk(void* s, int nx, int ny, double centre, double dz, double radius, hipStream_t st);
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st);
+// per-job final error reduction of the white-field row pass (grid = njobs)
+void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st);
 void k_final_max(const double* partials, int n, double* out, hipStream_t st);
 void k_final_sum1(const double* partials, int n, double* out, hipStream_t st);
 

@@ -1,0 +1,88 @@
+"""Batched iteration (nadia_batch): every job's estimate and errors are
+bit-identical to iterating the same job alone, whatever else is in the
+batch (uneven sizes, kinds, algorithms, precisions)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def planar_job(nd, synth, n, seed, alg, double=True, beta=0.9, beam=False):
+    d = synth.planar_block(n=n, block=3 * n // 8, support=7 * n // 16, seed_amp=seed, seed_poisson=seed,
+                           seed_phase=seed + 1)
+    cdt = np.complex128 if double else np.complex64
+    est = nd.Complex2D(n, n, double=double, data=d["x0"].astype(cdt))
+    g = nd.PlanarCDI(est)
+    g.set_support(d["support"])
+    g.set_intensity(d["intensity"])
+    if beam:
+        b = np.ones((n, n))
+        b[n // 2 - 2:n // 2 + 2, n // 2 - 2:n // 2 + 2] = 0
+        g.set_beam_stop(b)
+    g.set_relaxation_parameter(beta)
+    g.set_algorithm(alg)
+    return est, g
+
+
+def specs(nd):
+    return [(256, 1, nd.HIO, True), (256, 2, nd.RAAR, True), (384, 3, nd.ER, True), (256, 4, nd.DM, True),
+            (512, 5, nd.HIO, False), (256, 6, nd.HIO, False), (768, 7, nd.RAAR, False)]
+
+
+def test_batch_matches_alone(nd, synth):
+    alone = []
+    for n, seed, alg, double in specs(nd):
+        est, g = planar_job(nd, synth, n, seed, alg, double, beam=seed % 2 == 0)
+        e = g.iterate(7)
+        alone.append((est.get(), e))
+    jobs = [planar_job(nd, synth, n, seed, alg, double, beam=seed % 2 == 0) for n, seed, alg, double in specs(nd)]
+    b = nd.Batch([g for _, g in jobs])
+    e = np.vstack([b.iterate(3), b.iterate(4)])
+    assert e.shape == (7, len(jobs))
+    for j, (est, g) in enumerate(jobs):
+        x0, e0 = alone[j]
+        assert np.array_equal(est.get(), x0), j
+        assert np.array_equal(e[:, j], e0), j
+        assert g.get_error() == e0[-1]
+
+
+def test_batch_membership_changes(nd, synth):
+    """Jobs join and leave between iterations (continuous batching): a job's
+    trajectory does not depend on when others come and go."""
+    est_a, a = planar_job(nd, synth, 256, 11, nd.HIO)
+    ref_est, ref = planar_job(nd, synth, 256, 11, nd.HIO)
+    ref_e = ref.iterate(6)
+    b = nd.Batch([a])
+    e = list(b.iterate(2)[:, 0])
+    est_b, bjob = planar_job(nd, synth, 256, 12, nd.RAAR)
+    b.add(bjob)
+    e += list(b.iterate(2)[:, 0])
+    b.remove(bjob)
+    e += list(b.iterate(2)[:, 0])
+    assert np.array_equal(np.array(e), ref_e)
+    assert np.array_equal(est_a.get(), ref_est.get())
+
+
+def test_batch_fresnel_and_wf(nd, synth):
+    d = synth.fresnel_config(n=128, pupil_diameter=160e-6 / 8, disk_radius=4.5, ndisks=3)
+    def fres():
+        est = nd.Complex2D(128, 128)
+        g = nd.FresnelCDI(est, d["white_field"], d["wavelength"], d["zfd"], d["zfs"], d["pixel"], 1.0)
+        g.set_support(d["support"])
+        g.set_intensity(d["intensity"])
+        g.initialise_estimate(0)
+        return est, g
+    def wf():
+        est = nd.Complex2D(128, 128)
+        g = nd.FresnelCDI_WF(est, d["wavelength"], d["z12"], d["z23"], d["pixel"])
+        g.set_support(d["pupil_diameter"])
+        g.set_intensity(d["white_field_intensity"])
+        g.initialise_estimate(0)
+        return est, g
+    (e1, g1), (e2, g2) = fres(), wf()
+    r1, r2 = g1.iterate(5), g2.iterate(5)
+    (f1, h1), (f2, h2) = fres(), wf()
+    b = nd.Batch([h1, h2])
+    e = b.iterate(5)
+    assert np.array_equal(e[:, 0], r1) and np.array_equal(e[:, 1], r2)
+    assert np.array_equal(f1.get(), e1.get()) and np.array_equal(f2.get(), e2.get())
