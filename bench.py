@@ -2,9 +2,17 @@
 
 A step is one HIO iteration (beta 0.9) of the 2048^2 star-object workload
 (beam stop 16x16, support 900x900) with the estimate, intensity and support
-resident in HBM.  `value` is the FP32-bulk fused path (FP64 error sums); the
-FP64 paths (plain = one kernel per reference operation, fused) are timed
-beside it on the same input, and the CPU oracle on a bounded sample.
+resident in HBM.  `value` is the FP32-bulk fused path (FP64 error sums),
+batched; the FP64 paths (plain = one kernel per reference operation, fused)
+are timed beside it on the same input, and the CPU oracle on a bounded
+sample.  Further legs (each can be skipped):
+  batched64  64 independent RAAR jobs of sizes {256..1024}^2 (FP32), eager
+             launches and graph replays
+  served     a Poisson request stream through the continuous-batching loop
+             (p50/p99 latency, aggregate iterations/s)
+  accuracy   final-object error of the FP32 path against the FP64 path for
+             the 2048^2 HIO + shrinkwrap + ER schedule, the 64-job batch and
+             the 1024^2 Fresnel workload
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -70,6 +78,169 @@ def time_iterations(nd, p, steps, warmup):
     return time.perf_counter() - t0
 
 
+def rel_l2(a, b):
+    a = np.asarray(a, np.complex128)
+    b = np.asarray(b, np.complex128)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def leg_accuracy_planar(nd, synth, data, hio=1000, every=20, er=100):
+    """The 2048^2 schedule (HIO beta 0.9 with shrinkwrap every `every`
+    iterations, sigma 3 -> 1.5, threshold 0.1, then ER) in FP32 and FP64
+    from the same start; FP64 once more from a start perturbed by 1e-15
+    (relative) to show how far rounding alone moves this trajectory."""
+    ncalls = hio // every
+    out = {}
+    finals = {}
+    for tag, double, eps in (("fp32", False, 0.0), ("fp64", True, 0.0), ("fp64_perturbed", True, 1e-15)):
+        d = dict(data)
+        if eps:
+            rng = np.random.default_rng(99)
+            d["x0"] = data["x0"] * (1 + eps * rng.standard_normal(data["x0"].shape))
+        est, p = setup(nd, d, double, nd.PATH_FUSED)
+        p.iterate_async(1)  # tile-variant choice and first transform outside the clock
+        nd.synchronize()
+        est.set(d["x0"].astype(np.complex128 if double else np.complex64))
+        t0 = time.perf_counter()
+        for k in range(ncalls):
+            p.iterate_async(every)
+            p.apply_shrinkwrap(synth.shrinkwrap_sigma(k, ncalls), 0.1)
+        p.set_algorithm(nd.ER)
+        p.iterate_async(er)
+        nd.synchronize()
+        out[tag + "_s"] = round(time.perf_counter() - t0, 4)
+        out[tag + "_final_error"] = p.get_error()
+        finals[tag] = est.get()
+        del p, est
+    out["fp32_vs_fp64_rel_l2"] = rel_l2(finals["fp32"], finals["fp64"])
+    out["fp64_perturbed_vs_fp64_rel_l2"] = rel_l2(finals["fp64_perturbed"], finals["fp64"])
+    out["schedule"] = "%d HIO (beta 0.9) with shrinkwrap every %d (sigma 3->1.5, threshold 0.1), then %d ER" % (
+        hio, every, er)
+    return out
+
+
+def leg_batched(nd, synth, iters, graph_steps):
+    """64 independent RAAR (beta 0.85) jobs of sizes drawn from
+    {256,384,512,768,1024}^2: FP32 eager launches, FP32 graph replays
+    (`graph_steps` iterations per graph) and FP64 eager, each from the same
+    start for `iters` iterations in total; final-object error FP32 vs FP64."""
+    jobs = synth.batched_jobs()
+
+    def make(double):
+        cdt = np.complex128 if double else np.complex64
+        out = []
+        for d in jobs:
+            n = d["n"]
+            est = nd.Complex2D(n, n, double=double, data=d["x0"].astype(cdt))
+            g = nd.PlanarCDI(est)
+            g.set_support(d["support"])
+            g.set_intensity(d["intensity"])
+            g.set_relaxation_parameter(0.85)
+            g.set_algorithm(nd.RAAR)
+            out.append((est, g))
+        return out
+
+    px = sum(d["n"] ** 2 for d in jobs)
+    warm = graph_steps
+    timed = iters - warm
+    res = {"jobs": len(jobs), "pixels_per_sweep": px, "iterations": iters, "timed_iterations": timed,
+           "graph_steps": graph_steps,
+           "sizes": {str(n): sum(1 for d in jobs if d["n"] == n) for n in sorted({d["n"] for d in jobs})}}
+    finals = {}
+    for tag, double, graph in (("fp32_eager", False, False), ("fp32_graph", False, True), ("fp64_eager", True, False)):
+        js = make(double)
+        b = nd.Batch([g for _, g in js])
+        if graph:
+            b.set_graph(graph_steps)
+        b.iterate_async(warm)
+        nd.synchronize()
+        t0 = time.perf_counter()
+        b.iterate_async(timed)
+        nd.synchronize()
+        dt = time.perf_counter() - t0
+        bpp = pass_bytes(double)["iter"]
+        res[tag] = {"job_iter_s": round(len(jobs) * timed / dt, 1), "ms_per_sweep": round(dt / timed * 1e3, 4),
+                    "algorithmic_GBs": round(bpp * px * timed / dt / 1e9, 1)}
+        if graph:
+            res[tag]["graph_replays"] = b.graph_replays()
+        finals[tag] = [est.get() for est, _ in js]
+        del b, js
+    res["graph_bit_identical_to_eager"] = all(np.array_equal(a, b) for a, b in
+                                              zip(finals["fp32_eager"], finals["fp32_graph"]))
+    errs = [rel_l2(a, b) for a, b in zip(finals["fp32_graph"], finals["fp64_eager"])]
+    res["fp32_vs_fp64_rel_l2"] = {"median": float(np.median(errs)), "max": float(np.max(errs)),
+                                  "jobs_le_1e-4": int(sum(e <= 1e-4 for e in errs))}
+    return res
+
+
+def leg_fresnel(nd, synth, n=1024, wf_iters=50, er_iters=300):
+    """Fresnel workload: white field recovered by FresnelCDI_WF iterations on
+    its intensity, then FresnelCDI ER on the object intensity, FP32 and FP64;
+    iterations/s and final transmission-function error FP32 vs FP64."""
+    d = synth.fresnel_config(n=n)
+    out = {"n": n, "wf_iterations": wf_iters, "er_iterations": er_iters}
+    fin = {}
+    for tag, double in (("fp32", False), ("fp64", True)):
+        wfe = nd.Complex2D(n, n, double=double)
+        w = nd.FresnelCDI_WF(wfe, d["wavelength"], d["z12"], d["z23"], d["pixel"])
+        w.set_support(d["pupil_diameter"])
+        w.set_intensity(d["white_field_intensity"])
+        w.initialise_estimate(0)
+        t0 = time.perf_counter()
+        w.iterate_async(wf_iters)
+        nd.synchronize()
+        t_wf = time.perf_counter() - t0
+        wf = wfe.get()
+        est = nd.Complex2D(n, n, double=double)
+        f = nd.FresnelCDI(est, wf, d["wavelength"], d["zfd"], d["zfs"], d["pixel"], 1.0)
+        f.set_support(d["support"])
+        f.set_intensity(d["intensity"])
+        f.initialise_estimate(0)
+        f.iterate_async(1)
+        nd.synchronize()
+        f.initialise_estimate(0)
+        t0 = time.perf_counter()
+        f.iterate_async(er_iters)
+        nd.synchronize()
+        t_er = time.perf_counter() - t0
+        out[tag] = {"wf_iter_s": round(wf_iters / t_wf, 1), "er_iter_s": round(er_iters / t_er, 1),
+                    "final_error": f.get_error()}
+        fin[tag] = (wf, f.get_transmission_function())
+        del f, est, w, wfe
+    out["white_field_fp32_vs_fp64_rel_l2"] = rel_l2(fin["fp32"][0], fin["fp64"][0])
+    out["transmission_fp32_vs_fp64_rel_l2"] = rel_l2(fin["fp32"][1], fin["fp64"][1])
+    return out
+
+
+def leg_served(nd, synth, count, rate, budget):
+    """Poisson request stream through the continuous-batching loop (FP32
+    bulk, FP64 reductions), admission by an HBM budget."""
+    serve = importlib.import_module("nadia-coecxs_amd.serve")
+    reqs = serve.poisson_stream(count=count, rate=rate)
+    sizes = sorted({r.n for r in reqs})
+    t0 = time.perf_counter()
+    inputs = serve.input_pool(synth, sizes, per_size=2)
+    t_gen = time.perf_counter() - t0
+    be = serve.DeviceBackend(nd, inputs)
+    # one short job per size first: tile-variant choice and twiddle tables
+    # happen outside the measured stream
+    serve.Server(be, hbm_budget=budget).run(
+        [serve.Request(-1 - i, 0.0, n, serve.HIO, 0.9, 20, False, 0) for i, n in enumerate(sizes)])
+    srv = serve.Server(be, hbm_budget=budget)
+    done = srv.run(reqs)
+    s = serve.Server.summary(done, srv.elapsed)
+    s.update({"rate_req_s": rate, "hbm_budget_GB": budget / 1e9, "peak_active": srv.peak_active,
+              "peak_bytes_GB": round(srv.peak_bytes / 1e9, 3), "ticks": srv.ticks,
+              "input_pool_s": round(t_gen, 1),
+              "p50_ms": round(s["p50_ms"], 3), "p99_ms": round(s["p99_ms"], 3),
+              "aggregate_iter_s": round(s["aggregate_iter_s"], 1)})
+    by = {}
+    for c in done:
+        by.setdefault(c.n, []).append(c.latency * 1e3)
+    s["p50_ms_by_size"] = {str(n): round(float(np.percentile(v, 50)), 3) for n, v in sorted(by.items())}
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +251,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp32-only", action="store_true",
                     help="run only the FP32 fused iterations (for profiler runs)")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the FP32-vs-FP64 schedule runs")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 64-job batch")
+    ap.add_argument("--batched-iters", type=int, default=500)
+    ap.add_argument("--graph-steps", type=int, default=50)
+    ap.add_argument("--no-fresnel", action="store_true", help="skip the 1024^2 Fresnel workload")
+    ap.add_argument("--no-served", action="store_true", help="skip the request stream")
+    ap.add_argument("--serve-requests", type=int, default=2000)
+    ap.add_argument("--serve-rate", type=float, default=40.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="instead of the benchmark, run K device copies of a 2048^2 complex64 field "
                          "(32 MiB read + 32 MiB written each) to calibrate profiler byte counters")
@@ -167,6 +347,20 @@ def main():
     print("bench: fp64 fused %.1f iter/s" % fused64, file=sys.stderr, flush=True)
     del p64, est64
 
+    extra = {}
+    if not args.no_accuracy:
+        extra["accuracy_2048"] = leg_accuracy_planar(nd, synth, data)
+        print("bench: accuracy %s" % json.dumps(extra["accuracy_2048"]), file=sys.stderr, flush=True)
+    if not args.no_batched:
+        extra["batched64"] = leg_batched(nd, synth, args.batched_iters, args.graph_steps)
+        print("bench: batched64 %s" % json.dumps(extra["batched64"]), file=sys.stderr, flush=True)
+    if not args.no_fresnel:
+        extra["fresnel1024"] = leg_fresnel(nd, synth)
+        print("bench: fresnel %s" % json.dumps(extra["fresnel1024"]), file=sys.stderr, flush=True)
+    if not args.no_served:
+        extra["served"] = leg_served(nd, synth, args.serve_requests, args.serve_rate, args.serve_budget_gb * 1e9)
+        print("bench: served %s" % json.dumps(extra["served"]), file=sys.stderr, flush=True)
+
     # ---- CPU baseline: the oracle on the host, bounded sample -----------
     cpu = None
     if not args.no_cpu:
@@ -207,6 +401,10 @@ def main():
         "final_error_fp32": err32,
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    out.update(extra)
+    if "served" in extra:
+        out["served_p50_ms"] = extra["served"]["p50_ms"]
+        out["served_p99_ms"] = extra["served"]["p99_ms"]
     print(json.dumps(out))
 
 

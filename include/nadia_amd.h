@@ -24,6 +24,8 @@
 #ifndef NADIA_AMD_H
 #define NADIA_AMD_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -183,6 +185,13 @@ int nadia_wf_setup(nadia_recon* r, double wavelength, double zone_focal_length,
 /* set_support(z_diameter, size) (src/FresnelCDI_WF.c++:177-196) */
 int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size);
 
+/* Device memory (bytes) a reconstruction holds besides its estimate field,
+ * and an estimate for one not yet created, estimate field included (fused
+ * path, beam stop set): the admission figure of a request loop.  No
+ * counterpart in the reference. */
+int nadia_recon_device_bytes(const nadia_recon* r, size_t* out);
+int nadia_recon_bytes_estimate(int nx, int ny, int precision, int kind, int n_best, size_t* out);
+
 /* ---- batches (no counterpart in the reference) ----
  * A batch iterates many reconstructions together: those of the same
  * precision, kind and size share every launch of the fused passes, so small
@@ -200,6 +209,20 @@ int nadia_batch_size(const nadia_batch* b, int* n);
  * receives n x size values, errors[k*size + j] = get_error() of job j (in the
  * order added) after iteration k. */
 int nadia_batch_iterate(nadia_batch* b, int n, double* errors);
+/* counts[j] iterations of job j (in the order added; 0 leaves it untouched):
+ * job j takes part in the first counts[j] launches of its group, so jobs at
+ * different points of their schedules share launches (continuous batching).
+ * Each job's result is the same as nadia_recon_iterate(r, counts[j], NULL). */
+int nadia_batch_iterate_each(nadia_batch* b, const int* counts);
+/* Graph mode: with steps > 0, a nadia_batch_iterate of n >= steps iterations
+ * runs as replays of one captured hipGraph holding `steps` iterations of
+ * every launch group (groups as parallel branches); the graph is rebuilt
+ * only when the batch's launches change.  0 (default) launches eagerly.
+ * Results are bit-identical to eager launches.  Applies to batches whose
+ * jobs all take the fused path and keep no n-best snapshots. */
+int nadia_batch_set_graph(nadia_batch* b, int steps);
+/* graph replays run so far (for tests and measurement) */
+int nadia_batch_graph_replays(const nadia_batch* b, long long* n);
 
 /* ---- tuning and measurement ---- */
 

@@ -105,6 +105,11 @@ def _sig(lib):
         "nadia_batch_remove": (I, [P, P]),
         "nadia_batch_size": (I, [P, IP]),
         "nadia_batch_iterate": (I, [P, I, DP]),
+        "nadia_batch_iterate_each": (I, [P, IP]),
+        "nadia_batch_set_graph": (I, [P, I]),
+        "nadia_batch_graph_replays": (I, [P, LP]),
+        "nadia_recon_device_bytes": (I, [P, ctypes.POINTER(ctypes.c_size_t)]),
+        "nadia_recon_bytes_estimate": (I, [I, I, I, I, I, ctypes.POINTER(ctypes.c_size_t)]),
         "nadia_recon_set_tile_variants": (I, [P, I, I]),
         "nadia_recon_get_tile_variants": (I, [P, IP, IP]),
         "nadia_recon_set_timing": (I, [P, I]),
@@ -346,6 +351,12 @@ class BaseCDI:
     def set_timing(self, on=True):
         _check(library().nadia_recon_set_timing(self.h, int(bool(on))))
 
+    def device_bytes(self):
+        """Device bytes this reconstruction holds besides its estimate."""
+        v = ctypes.c_size_t()
+        _check(library().nadia_recon_device_bytes(self.h, ctypes.byref(v)))
+        return v.value
+
     def pass_times(self):
         ms = (ctypes.c_double * 2)()
         cnt = (ctypes.c_longlong * 2)()
@@ -467,4 +478,31 @@ class Batch:
 
     def iterate_async(self, n=1):
         _check(library().nadia_batch_iterate(self.h, int(n), None))
+
+    def iterate_each(self, counts):
+        """counts[j] iterations of job j (jobs in the order added); queued
+        without waiting, like iterate_async."""
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        if c.shape != (len(self.jobs),):
+            raise ValueError("need one count per job")
+        if self.jobs:
+            _check(library().nadia_batch_iterate_each(self.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+
+    def set_graph(self, steps):
+        """Run uniform iterate() calls as replays of a captured graph of
+        `steps` iterations (0: eager launches)."""
+        _check(library().nadia_batch_set_graph(self.h, int(steps)))
+
+    def graph_replays(self):
+        v = ctypes.c_longlong()
+        _check(library().nadia_batch_graph_replays(self.h, ctypes.byref(v)))
+        return v.value
+
+
+def bytes_estimate(nx, ny, double=False, kind=PLANAR, n_best=0):
+    """Device bytes a reconstruction of this shape holds, estimate included."""
+    v = ctypes.c_size_t()
+    _check(library().nadia_recon_bytes_estimate(int(nx), int(ny), FP64 if double else FP32, int(kind), int(n_best),
+                                                ctypes.byref(v)))
+    return v.value
 

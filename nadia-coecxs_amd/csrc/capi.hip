@@ -14,6 +14,7 @@
 // iterations.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -59,6 +60,11 @@ int fail(int code, const char* fmt, ...) {
 struct DeviceCtx {
   hipStream_t stream = nullptr;
   std::map<std::pair<int, int>, void*> twiddles;  // (n, precision) -> table
+  // side streams (default priority) and events that fork independent groups
+  // of a batch off `stream` while a graph is captured
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
 };
 
 std::mutex g_mu;
@@ -193,6 +199,13 @@ struct nadia_batch {
   std::vector<nadia_recon*> jobs;
   JobDesc* djobs;  // device descriptors, grouped
   int cap;
+  // graph mode: uniform sweeps run as replays of a captured graph of
+  // `gsteps` iterations; gsig describes the launches it was captured for
+  int graph_steps;
+  hipGraphExec_t gexec;
+  int gsteps;
+  std::vector<long long> gsig;
+  long long graph_replays;
 };
 
 namespace {
@@ -356,6 +369,8 @@ int iterate_plain(nadia_recon* r, double* slot) {
     if ((rc = op_project(r, x, slot))) return rc;
     return op_support(r, x);
   }
+  for (int n = 0; n < 4; n++)
+    if (r->alg[n] != 0 && !r->t[n] && (rc = alloc(&r->t[n], r->npix * r->esz))) return rc;
   void *tPSF = r->t[0], *tPFS = r->t[1], *tPS = r->t[2], *tPF = r->t[3];
   const double* a = r->alg;
   if (a[A_PFS] != 0) {
@@ -526,16 +541,16 @@ GroupKey group_key(const nadia_recon* r) { return GroupKey{r->prec, r->kind, r->
 
 // Queue one fused iteration of `njobs` reconstructions sharing key g, their
 // descriptors in djobs (device); error ring slot `iter`; column / row tile
-// variants cv / rv.  PlanarCDI and FresnelCDI: column pass (modulus
+// variants cv / rv; on stream st (default: the device's stream).  PlanarCDI and FresnelCDI: column pass (modulus
 // projection) + row pass (update).  FresnelCDI_WF: four passes and a final
 // error reduction.  evs (may be null) receives per-pass timing events.
 template <typename T>
 int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njobs, int iter, int cv, int rv,
-                 nadia_recon* timer, std::vector<Ev>* evs) {
+                 nadia_recon* timer, std::vector<Ev>* evs, hipStream_t st = nullptr) {
   void* twy = twiddles(c, g.ny, g.prec, true);
   void* twx = twiddles(c, g.nx, g.prec, false);
   if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
-  hipStream_t st = c->stream;
+  if (!st) st = c->stream;
   ColArgs ca{};
   ca.jobs = djobs;
   ca.nfields = g.two ? 2 : 1;
@@ -1027,6 +1042,48 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   return 0;
 }
 
+}  // extern "C"
+
+namespace {
+// Device bytes owned by a reconstruction of npix pixels (not counting its
+// estimate field): S, Q, conv, rtmp (real), W (2 complex fields) and the small
+// per-job buffers; beam stop, unfused temporaries, Fresnel fields and n-best
+// snapshots when present.
+size_t base_bytes(long long npix, size_t esz) {
+  const size_t cb = npix * esz, rb = npix * (esz / 2);
+  return 4 * rb + 2 * cb + 64 * 8 + sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16) +
+         sizeof(double) * (4096 + 8) + sizeof(double) * 16 + sizeof(JobDesc) * 2 + sizeof(double) * 64;
+}
+}  // namespace
+
+extern "C" {
+
+int nadia_recon_device_bytes(const nadia_recon* r, size_t* out) {
+  if (!r || !out) return fail(NADIA_EINVAL, "null argument");
+  const size_t cb = r->npix * r->esz, rb = r->npix * r->rsz;
+  size_t b = base_bytes(r->npix, r->esz);
+  if (r->beam) b += rb;
+  for (void* t : r->t)
+    if (t) b += cb;
+  if (r->L) b += cb;
+  if (r->Ls) b += cb;
+  b += r->best.size() * cb;
+  *out = b;
+  return 0;
+}
+
+int nadia_recon_bytes_estimate(int nx, int ny, int precision, int kind, int n_best, size_t* out) {
+  if (!out || nx <= 0 || ny <= 0 || n_best < 0) return fail(NADIA_EINVAL, "bad argument");
+  if (precision != NADIA_FP32 && precision != NADIA_FP64) return fail(NADIA_EINVAL, "bad precision");
+  const long long npix = (long long)nx * ny;
+  const size_t esz = precision == NADIA_FP64 ? 16 : 8;
+  const size_t cb = npix * esz;
+  size_t b = cb + base_bytes(npix, esz) + npix * (esz / 2) + (size_t)n_best * cb;
+  if (kind == NADIA_FRESNEL || kind == NADIA_FRESNEL_WF) b += 2 * cb;
+  *out = b;
+  return 0;
+}
+
 int nadia_recon_destroy(nadia_recon* r) {
   if (!r) return 0;
   (void)hipStreamSynchronize(S_(r));
@@ -1089,16 +1146,14 @@ int nadia_recon_set_custom_algorithm(nadia_recon* r, const double m[10]) {
   r->alg[A_PS] = -m[2] - m[5] - m[7] + m[9];
   r->alg[A_PF] = -m[1] - m[4] + m[7] + m[8];
   r->alg[A_PI] = -m[3] - m[6] - m[8] - m[9];
-  // reallocate_temp_complex_memory (src/BaseCDI.c++:257-275)
+  // reallocate_temp_complex_memory (src/BaseCDI.c++:257-275): temporaries of
+  // terms that vanished are freed here; the unfused path allocates the ones it
+  // needs on first use (the fused path never needs them)
   for (int n = 0; n < 4; n++) {
     if (r->alg[n] == 0 && r->t[n]) {
       (void)hipStreamSynchronize(S_(r));
       (void)hipFree(r->t[n]);
       r->t[n] = nullptr;
-    }
-    if (r->alg[n] != 0 && !r->t[n]) {
-      int rc = alloc(&r->t[n], r->npix * r->esz);
-      if (rc) return rc;
     }
   }
   r->algorithm = NADIA_CUSTOM;
@@ -1587,6 +1642,7 @@ int nadia_batch_destroy(nadia_batch* b) {
   if (!b) return 0;
   DeviceCtx* c = dev_ctx(b->dev);
   if (c) (void)hipStreamSynchronize(c->stream);
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->djobs) (void)hipFree(b->djobs);
   delete b;
   return 0;
@@ -1617,92 +1673,254 @@ int nadia_batch_size(const nadia_batch* b, int* n) {
   return 0;
 }
 
-int nadia_batch_iterate(nadia_batch* b, int n, double* errors) {
+int nadia_batch_set_graph(nadia_batch* b, int steps) {
   if (!b) return fail(NADIA_EINVAL, "null batch");
+  if (steps < 0 || steps > 4096) return fail(NADIA_EINVAL, "graph length %d outside 0..4096", steps);
+  b->graph_steps = steps;
+  return 0;
+}
+
+int nadia_batch_graph_replays(const nadia_batch* b, long long* n) {
+  if (!b || !n) return fail(NADIA_EINVAL, "null argument");
+  *n = b->graph_replays;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// One launch group of a batch chunk: jobs of one GroupKey, their descriptors
+// at d (device), sorted by the iterations they still run in this chunk
+// (descending), so iteration k launches the prefix of jobs with rem > k.
+struct BatchLaunch {
+  GroupKey g;
+  JobDesc* d;
+  int cv, rv;
+  std::vector<int> rem;
+  int active(int k) const {
+    int a = 0;
+    while (a < (int)rem.size() && rem[a] > k) a++;
+    return a;
+  }
+};
+
+int ensure_side(DeviceCtx* c, int n) {
+  if (!c->fork) HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+  while ((int)c->side.size() < n) {
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    c->side.push_back(st);
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->join.push_back(e);
+  }
+  return 0;
+}
+
+// Capture `steps` iterations of every launch group into a graph: each group
+// is an independent chain of passes (groups touch disjoint jobs), forked off
+// the device stream so the graph holds them as parallel branches.
+int capture_batch_graph(nadia_batch* b, DeviceCtx* c, const std::vector<BatchLaunch>& launches, int steps) {
+  std::vector<long long> sig{steps, (long long)(intptr_t)b->djobs};
+  for (const BatchLaunch& L : launches) {
+    long long v[] = {L.g.prec, L.g.kind, L.g.nx, L.g.ny, L.g.two, (long long)(L.d - b->djobs),
+                     (long long)L.rem.size(), L.cv, L.rv};
+    sig.insert(sig.end(), v, v + 9);
+  }
+  if (b->gexec && sig == b->gsig) return 0;
+  if (b->gexec) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+  }
+  const int nl = (int)launches.size();
+  int rc;
+  if ((rc = ensure_side(c, nl - 1))) return rc;
+  hipStream_t st = c->stream;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  rc = 0;
+  if (nl > 1 && hipEventRecord(c->fork, st) != hipSuccess) rc = fail(NADIA_EHIP, "event record in capture failed");
+  for (int i = 0; i < nl && !rc; ++i) {
+    const BatchLaunch& L = launches[i];
+    hipStream_t s2 = i == 0 ? st : c->side[i - 1];
+    if (i > 0 && hipStreamWaitEvent(s2, c->fork, 0) != hipSuccess) {
+      rc = fail(NADIA_EHIP, "stream wait in capture failed");
+      break;
+    }
+    for (int k = 0; k < steps && !rc; ++k)
+      DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, (int)L.rem.size(), k, L.cv, L.rv, nullptr, nullptr, s2));
+    if (!rc && i > 0 && hipEventRecord(c->join[i - 1], s2) != hipSuccess)
+      rc = fail(NADIA_EHIP, "event record in capture failed");
+  }
+  for (int i = 1; i < nl && !rc; ++i)
+    if (hipStreamWaitEvent(st, c->join[i - 1], 0) != hipSuccess) rc = fail(NADIA_EHIP, "stream join in capture failed");
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(st, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(NADIA_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&b->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    b->gexec = nullptr;
+    return fail(NADIA_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  b->gsig = sig;
+  return 0;
+}
+
+// counts[j]: iterations of job j (in the order added).  errors (only when
+// every count is equal) receives errors[k*size + j].
+int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
   const int nj = (int)b->jobs.size();
-  if (n <= 0 || nj == 0) return 0;
+  int nmax = 0;
+  bool uniform = true;
+  for (int j = 0; j < nj; ++j) {
+    if (counts[j] < 0) return fail(NADIA_EINVAL, "negative iteration count for job %d", j);
+    if (counts[j] > nmax) nmax = counts[j];
+    if (counts[j] != counts[0]) uniform = false;
+  }
+  if (nmax <= 0) return 0;
+  if (!uniform) errors = nullptr;
   DeviceCtx* c = dev_ctx(b->dev);
   if (!c) return fail(NADIA_EHIP, "stream creation failed");
   int rc;
   bool per_iteration = false;
-  for (nadia_recon* r : b->jobs) {
+  for (int j = 0; j < nj; ++j) {
+    nadia_recon* r = b->jobs[j];
+    if (counts[j] == 0) continue;
     if (!r->has_intensity) return fail(NADIA_ESTATE, "a reconstruction in the batch has no intensity");
     if ((rc = need_setup(r))) return rc;
     if (r->n_best > 0 && r->kind != NADIA_FRESNEL_WF) per_iteration = true;
   }
   if (b->cap < nj) {
+    if (b->gexec) {  // the graph addresses the old descriptor array
+      HIPCHK(hipStreamSynchronize(c->stream));
+      (void)hipGraphExecDestroy(b->gexec);
+      b->gexec = nullptr;
+    }
     if (b->djobs) (void)hipFree(b->djobs);
     b->djobs = nullptr;
     if ((rc = alloc((void**)&b->djobs, sizeof(JobDesc) * nj))) return rc;
     b->cap = nj;
   }
+  std::vector<double> tmp;
+  // copy the ring slots 0..m-1 of every job to errors rows base..base+m-1
+  auto emit = [&](int base, int m) -> int {
+    if (!errors) return 0;
+    tmp.resize(m);
+    for (int j = 0; j < nj; ++j) {
+      nadia_recon* r = b->jobs[j];
+      if (has_projection(r)) {
+        HIPCHK(hipMemcpyAsync(tmp.data(), r->errs, m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+      } else {
+        double v = 0;
+        if ((rc = fetch_error(r, &v))) return rc;
+        for (int k = 0; k < m; ++k) tmp[k] = v;
+      }
+      for (int k = 0; k < m; ++k) errors[(size_t)(base + k) * nj + j] = tmp[k];
+    }
+    return 0;
+  };
   int done = 0;
-  while (done < n) {
-    int chunk = n - done;
+  while (done < nmax) {
+    int chunk = nmax - done;
     if (chunk > 4096) chunk = 4096;
     if (per_iteration) chunk = 1;
+    std::vector<int> rem(nj);
+    for (int j = 0; j < nj; ++j) rem[j] = std::max(0, std::min(chunk, counts[j] - done));
     // group the fused-path jobs; the rest run the unfused path one by one
-    std::map<GroupKey, std::vector<nadia_recon*>> groups;
-    std::vector<nadia_recon*> plain;
-    for (nadia_recon* r : b->jobs) {
-      if (fused_ok(r)) groups[group_key(r)].push_back(r);
-      else plain.push_back(r);
+    std::map<GroupKey, std::vector<int>> groups;
+    std::vector<int> plain;
+    for (int j = 0; j < nj; ++j) {
+      if (rem[j] == 0) continue;
+      if (fused_ok(b->jobs[j])) groups[group_key(b->jobs[j])].push_back(j);
+      else plain.push_back(j);
     }
-    struct Launch {
-      GroupKey g;
-      JobDesc* d;
-      int count, cv, rv;
-    };
-    std::vector<Launch> launches;
+    std::vector<BatchLaunch> launches;
     std::vector<JobDesc> host;
     host.reserve(nj);
     for (auto& kv : groups) {
-      for (nadia_recon* r : kv.second) {
+      std::stable_sort(kv.second.begin(), kv.second.end(), [&](int x, int y) { return rem[x] > rem[y]; });
+      for (int j : kv.second) {
+        nadia_recon* r = b->jobs[j];
         DISPATCH(r->prec, rc = fused_ready<T>(r));
         if (rc) return rc;
       }
-      Launch L{kv.first, b->djobs + host.size(), (int)kv.second.size(), kv.second[0]->col_v, kv.second[0]->row_v};
-      for (nadia_recon* r : kv.second) host.push_back(make_jobdesc(r, r->est->d, r->W, r->errs));
-      launches.push_back(L);
+      nadia_recon* r0 = b->jobs[kv.second[0]];
+      BatchLaunch L{kv.first, b->djobs + host.size(), r0->col_v, r0->row_v, {}};
+      for (int j : kv.second) {
+        host.push_back(make_jobdesc(b->jobs[j], b->jobs[j]->est->d, b->jobs[j]->W, b->jobs[j]->errs));
+        L.rem.push_back(rem[j]);
+      }
+      launches.push_back(std::move(L));
     }
     if (!host.empty() && (rc = upload_jobs(c->stream, b->djobs, host.data(), (int)host.size()))) return rc;
-    for (int k = 0; k < chunk; ++k) {
-      for (const Launch& L : launches) {
-        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, L.count, k, L.cv, L.rv, nullptr, nullptr));
+    // graph replays for the uniform, fused-only part; the rest eagerly
+    const int G = b->graph_steps;
+    const bool graph = G > 0 && uniform && plain.empty() && !per_iteration && !launches.empty() && chunk >= G;
+    int eager_from = 0, last_slot = chunk - 1;
+    if (graph) {
+      if ((rc = capture_batch_graph(b, c, launches, G))) return rc;
+      const int reps = chunk / G;
+      for (int q = 0; q < reps; ++q) {
+        HIPCHK(hipGraphLaunch(b->gexec, c->stream));
+        b->graph_replays++;
+        if ((rc = emit(done + q * G, G))) return rc;
+      }
+      eager_from = reps * G;
+      last_slot = eager_from < chunk ? chunk - eager_from - 1 : G - 1;
+    }
+    for (int k = eager_from; k < chunk; ++k) {
+      const int slot = k - eager_from;
+      for (const BatchLaunch& L : launches) {
+        const int a = L.active(k);
+        if (a == 0) continue;
+        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, slot, L.cv, L.rv, nullptr, nullptr));
         if (rc) return rc;
       }
-      for (nadia_recon* r : plain)
-        if ((rc = iterate_plain(r, r->errs + k))) return rc;
+      for (int j : plain)
+        if (rem[j] > k && (rc = iterate_plain(b->jobs[j], b->jobs[j]->errs + slot))) return rc;
     }
-    for (auto& kv : groups)
-      for (nadia_recon* r : kv.second) fused_done(r, chunk);
-    for (nadia_recon* r : plain) r->est->version++;
-    for (nadia_recon* r : b->jobs)
+    if (eager_from < chunk && (rc = emit(done + eager_from, chunk - eager_from))) return rc;
+    for (int j = 0; j < nj; ++j) {
+      if (rem[j] == 0) continue;
+      nadia_recon* r = b->jobs[j];
+      if (fused_ok(r)) fused_done(r, graph ? last_slot + 1 : rem[j]);
+      else r->est->version++;
       if (per_iteration && r->n_best > 0 && r->kind != NADIA_FRESNEL_WF && (rc = update_n_best(r))) return rc;
-    if (errors) {
-      std::vector<double> tmp(chunk);
-      for (int j = 0; j < nj; ++j) {
-        nadia_recon* r = b->jobs[j];
-        if (has_projection(r)) {
-          HIPCHK(hipMemcpyAsync(tmp.data(), r->errs, chunk * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-          HIPCHK(hipStreamSynchronize(c->stream));
-        } else {
-          double v = 0;
-          if ((rc = fetch_error(r, &v))) return rc;
-          for (int k = 0; k < chunk; ++k) tmp[k] = v;
-        }
-        for (int k = 0; k < chunk; ++k) errors[(size_t)(done + k) * nj + j] = tmp[k];
-      }
     }
     done += chunk;
-    if (done < n) HIPCHK(hipStreamSynchronize(c->stream));
+    if (done < nmax) HIPCHK(hipStreamSynchronize(c->stream));
   }
-  for (nadia_recon* r : b->jobs)
-    if (r->last_err && r->last_err != r->scal + 8) {
+  for (int j = 0; j < nj; ++j) {
+    nadia_recon* r = b->jobs[j];
+    if (counts[j] > 0 && r->last_err && r->last_err != r->scal + 8) {
       HIPCHK(hipMemcpyAsync(r->scal + 8, r->last_err, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
       r->last_err = r->scal + 8;
     }
+  }
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nadia_batch_iterate(nadia_batch* b, int n, double* errors) {
+  if (!b) return fail(NADIA_EINVAL, "null batch");
+  if (n <= 0 || b->jobs.empty()) return 0;
+  return batch_run(b, std::vector<int>(b->jobs.size(), n), errors);
+}
+
+int nadia_batch_iterate_each(nadia_batch* b, const int* counts) {
+  if (!b || (!counts && !b->jobs.empty())) return fail(NADIA_EINVAL, "null argument");
+  if (b->jobs.empty()) return 0;
+  return batch_run(b, std::vector<int>(counts, counts + b->jobs.size()), nullptr);
 }
 
 int nadia_recon_set_tile_variants(nadia_recon* r, int col_variant, int row_variant) {

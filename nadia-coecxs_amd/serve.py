@@ -1,0 +1,295 @@
+"""Request loop for reconstruction jobs with continuous batching.
+
+Frames of a beamline stream arrive as requests, each a PlanarCDI
+reconstruction with its own size, algorithm, iteration count and shrinkwrap
+schedule.  The server
+
+* admits arrived requests in arrival order while their device memory fits
+  the HBM budget (and a slot limit, if set); a request whose memory does not
+  fit waits, and so do the ones behind it;
+* keeps every admitted job's estimate, support, intensity and work fields
+  on the device between iterations;
+* iterates all admitted jobs together in ticks: one tick gives each job up
+  to `tick` iterations, cut at its next shrinkwrap point or its end, and
+  jobs of one precision and size share every launch of the fused passes
+  (nadia_batch_iterate_each);
+* retires a job as soon as its last iteration is done, and admits waiting
+  requests in the memory and slots it frees, while the others keep running.
+
+A job's result does not depend on what it ran beside: batched launches
+compute every job independently, with the same kernels and reductions as a
+job iterated alone.
+
+The device side is a backend object, so the scheduling logic runs (and is
+tested) with a simulated backend on the CPU; `DeviceBackend` is the one that
+drives the HIP library.
+"""
+import collections
+import dataclasses
+import time
+
+import numpy as np
+
+# algorithm ids as in NADIA's BaseCDI enum (NADIA/include/BaseCDI.h)
+ER, HIO, RAAR = 0, 3, 8
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    arrival: float      # seconds after the stream starts
+    n: int              # n x n pixels
+    algorithm: int
+    beta: float
+    iterations: int
+    shrinkwrap: bool    # shrinkwrap every `every` iterations
+    seed: int
+
+
+@dataclasses.dataclass
+class Completed:
+    rid: int
+    n: int
+    arrival: float
+    admitted: float
+    finished: float
+    iterations: int
+    error: float
+    result: object = None
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def poisson_stream(count=2000, rate=40.0, sizes=(256, 512, 1024, 2048), probs=(0.4, 0.3, 0.2, 0.1),
+                   iterations=(100, 1000), shrinkwrap_prob=0.5, seed=2024):
+    """Synthetic request stream: exponential inter-arrival times of mean
+    1/rate, size drawn from `sizes` with `probs`, algorithm uniform over
+    {ER, HIO beta 0.9, RAAR beta 0.85}, iteration count uniform on the closed
+    range `iterations`, shrinkwrap with probability `shrinkwrap_prob`.  All
+    draws from numpy's PCG64 generator seeded with `seed`."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    n = rng.choice(np.asarray(sizes), size=count, p=np.asarray(probs, dtype=float))
+    alg = rng.integers(0, 3, count)
+    its = rng.integers(iterations[0], iterations[1] + 1, count)
+    sw = rng.random(count) < shrinkwrap_prob
+    table = ((ER, 0.9), (HIO, 0.9), (RAAR, 0.85))
+    return [Request(i, float(t[i]), int(n[i]), table[alg[i]][0], table[alg[i]][1], int(its[i]), bool(sw[i]),
+                    int(seed * 100003 + i)) for i in range(count)]
+
+
+def shrinkwrap_sigma(call, ncalls, start=3.0, end=1.5):
+    """Gaussian width of shrinkwrap call `call` of `ncalls`: linear start->end."""
+    if ncalls <= 1:
+        return end
+    return start + (end - start) * min(call, ncalls - 1) / (ncalls - 1)
+
+
+def percentile(values, q):
+    """q-th percentile (linear interpolation between order statistics)."""
+    if not len(values):
+        return None
+    return float(np.percentile(np.asarray(values, dtype=float), q))
+
+
+class _Job:
+    __slots__ = ("req", "handle", "bytes", "done", "calls", "ncalls", "admitted")
+
+    def __init__(self, req, handle, nbytes, admitted, every):
+        self.req = req
+        self.handle = handle
+        self.bytes = nbytes
+        self.done = 0
+        self.calls = 0
+        self.ncalls = (req.iterations - 1) // every if req.shrinkwrap else 0
+        self.admitted = admitted
+
+
+class Server:
+    """Continuous-batching request loop (see the module docstring).
+
+    backend: object with bytes_for(req), start(req) -> handle,
+    run(handles, counts), shrinkwrap(handle, sigma, threshold), wait(),
+    finish(handle, keep) -> (error, result), release_idle(nbytes) -> freed
+    bytes (optional), held_bytes() -> bytes of idle cached jobs (optional).
+    """
+
+    def __init__(self, backend, hbm_budget=200e9, max_active=None, tick=50, every=50, threshold=0.1,
+                 keep_results=False):
+        self.backend = backend
+        self.budget = float(hbm_budget)
+        self.max_active = max_active
+        self.tick = int(tick)
+        self.every = int(every)
+        self.threshold = threshold
+        self.keep = keep_results
+        self.peak_bytes = 0
+        self.peak_active = 0
+        self.ticks = 0
+        self.iterations = 0
+
+    def _held(self):
+        f = getattr(self.backend, "held_bytes", None)
+        return f() if f else 0
+
+    def run(self, requests, clock=time.perf_counter, sleep=time.sleep):
+        pending = collections.deque(sorted(requests, key=lambda r: (r.arrival, r.rid)))
+        waiting = collections.deque()
+        active = []
+        used = 0
+        done = []
+        t0 = clock()
+        be = self.backend
+        while pending or waiting or active:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            # admission: arrival order, HBM budget, slot limit
+            while waiting and (self.max_active is None or len(active) < self.max_active):
+                req = waiting[0]
+                need = be.bytes_for(req)
+                if need > self.budget:
+                    raise ValueError("request %d needs %d bytes, more than the whole budget" % (req.rid, need))
+                if used + self._held() + need > self.budget and hasattr(be, "release_idle"):
+                    be.release_idle(used + self._held() + need - self.budget)
+                if used + self._held() + need > self.budget:
+                    break
+                waiting.popleft()
+                h = be.start(req)
+                active.append(_Job(req, h, need, clock() - t0, self.every))
+                used += need
+            self.peak_bytes = max(self.peak_bytes, used + self._held())
+            self.peak_active = max(self.peak_active, len(active))
+            if not active:
+                if pending:
+                    dt = pending[0].arrival - (clock() - t0)
+                    if dt > 0:
+                        sleep(dt)
+                continue
+            # one tick: up to `tick` iterations per job, cut at its next
+            # shrinkwrap point or its end
+            counts = []
+            for j in active:
+                left = j.req.iterations - j.done
+                c = min(self.tick, left)
+                if j.ncalls and j.calls < j.ncalls:
+                    c = min(c, (j.calls + 1) * self.every - j.done)
+                counts.append(c)
+            be.run([j.handle for j in active], counts)
+            self.ticks += 1
+            self.iterations += sum(counts)
+            for j, c in zip(active, counts):
+                j.done += c
+                if j.ncalls and j.calls < j.ncalls and j.done == (j.calls + 1) * self.every:
+                    be.shrinkwrap(j.handle, shrinkwrap_sigma(j.calls, j.ncalls), self.threshold)
+                    j.calls += 1
+            finished = [j for j in active if j.done >= j.req.iterations]
+            if finished:
+                be.wait()
+                tf = clock() - t0
+                for j in finished:
+                    err, res = be.finish(j.handle, self.keep)
+                    done.append(Completed(j.req.rid, j.req.n, j.req.arrival, j.admitted, tf, j.done, err, res))
+                    used -= j.bytes
+                active = [j for j in active if j.done < j.req.iterations]
+            else:
+                be.wait()
+        self.elapsed = clock() - t0
+        return done
+
+    @staticmethod
+    def summary(done, elapsed):
+        lat = [c.latency * 1e3 for c in done]
+        its = sum(c.iterations for c in done)
+        return {"requests": len(done), "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+                "mean_ms": float(np.mean(lat)) if lat else None, "elapsed_s": elapsed,
+                "aggregate_iter_s": its / elapsed if elapsed > 0 else None}
+
+
+class DeviceBackend:
+    """Backend over the HIP library: one nadia_batch, PlanarCDI jobs in FP32
+    bulk (FP64 reductions) or FP64.  Finished jobs' device objects are kept
+    per size and reused by later requests of that size; they count against
+    the budget (held_bytes) and are freed when admission needs the memory.
+
+    inputs(req) -> dict(intensity, support, x0[, beam_stop]) supplies each
+    request's data (host arrays)."""
+
+    def __init__(self, nd, inputs, double=False):
+        self.nd = nd
+        self.inputs = inputs
+        self.double = double
+        self.batch = nd.Batch()
+        self.idle = collections.defaultdict(list)   # n -> [(est, recon)]
+        self.idle_bytes = 0
+        self._bytes = {}
+
+    def bytes_for(self, req):
+        if req.n not in self._bytes:
+            self._bytes[req.n] = self.nd.bytes_estimate(req.n, req.n, double=self.double)
+        return self._bytes[req.n]
+
+    def held_bytes(self):
+        return self.idle_bytes
+
+    def release_idle(self, nbytes):
+        freed = 0
+        for n in sorted(self.idle, reverse=True):
+            while self.idle[n] and freed < nbytes:
+                est, p = self.idle[n].pop()
+                del p, est
+                freed += self._bytes[n]
+        self.idle_bytes -= freed
+        return freed
+
+    def start(self, req):
+        nd = self.nd
+        d = self.inputs(req)
+        cdt = np.complex128 if self.double else np.complex64
+        if self.idle[req.n]:
+            est, p = self.idle[req.n].pop()
+            self.idle_bytes -= self._bytes[req.n]
+            est.set(d["x0"].astype(cdt))
+        else:
+            est = nd.Complex2D(req.n, req.n, double=self.double, data=d["x0"].astype(cdt))
+            p = nd.PlanarCDI(est)
+        p.set_support(d["support"])
+        p.set_intensity(d["intensity"])
+        if "beam_stop" in d:
+            p.set_beam_stop(d["beam_stop"])
+        # set_relaxation_parameter before set_algorithm: selecting the
+        # algorithm a job already has does nothing (NADIA/src/BaseCDI.c++:194-195)
+        p.set_relaxation_parameter(req.beta)
+        p.set_algorithm(req.algorithm)
+        self.batch.add(p)
+        return (req, est, p)
+
+    def run(self, handles, counts):
+        # the batch holds exactly the active jobs, in admission order
+        order = {id(h[2]): c for h, c in zip(handles, counts)}
+        self.batch.iterate_each([order[id(p)] for p in self.batch.jobs])
+
+    def shrinkwrap(self, handle, sigma, threshold):
+        handle[2].apply_shrinkwrap(sigma, threshold)
+
+    def wait(self):
+        self.nd.synchronize()
+
+    def finish(self, handle, keep):
+        req, est, p = handle
+        err = p.get_error()
+        res = est.get() if keep else None
+        self.batch.remove(p)
+        self.idle[req.n].append((est, p))
+        self.idle_bytes += self._bytes[req.n]
+        return err, res
+
+
+def input_pool(synth, sizes, per_size=2, seed=7):
+    """Inputs as the 2048^2 star-object workload with radius and support
+    scaled by n/2048, `per_size` datasets per size (seeds seed + 10*k); a
+    request uses dataset req.seed % per_size of its size."""
+    pool = {n: [synth.planar_star(n=n, seed=seed + 10 * k) for k in range(per_size)] for n in sizes}
+    return lambda req: pool[req.n][req.seed % per_size]

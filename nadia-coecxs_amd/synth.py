@@ -106,6 +106,23 @@ def planar_star(n=2048, radius=400, support=900, beamstop=16, peak=1e8, ngauss=2
     return dict(intensity=I, support=sup, x0=x0, beam_stop=beam, obj=obj)
 
 
+def batched_jobs(count=64, sizes=(256, 384, 512, 768, 1024), seed=300):
+    """Independent block-object jobs of uneven sizes (the 64-job batch at the
+    defaults): size n drawn uniformly from `sizes` (seed), then each job as
+    planar_block with block side round(0.375 n) (96 px at 256), support side
+    round(0.4 n), seeds seed+1+j (amplitude and Poisson) and seed+10001+j
+    (initial phase).  Returns a list of dicts with an extra key n."""
+    ns = np.random.default_rng(seed).choice(np.asarray(sizes), size=count)
+    jobs = []
+    for j, n in enumerate(ns):
+        n = int(n)
+        d = planar_block(n=n, block=int(round(0.375 * n)), support=int(round(0.4 * n)), seed_amp=seed + 1 + j,
+                         seed_poisson=seed + 1 + j, seed_phase=seed + 10001 + j)
+        d["n"] = n
+        jobs.append(d)
+    return jobs
+
+
 def shrinkwrap_sigma(call, ncalls=50, start=3.0, end=1.5):
     """Gaussian width of shrinkwrap call `call` (0-based): linear start->end over ncalls."""
     if ncalls <= 1:

@@ -1,0 +1,155 @@
+"""Host logic of the request loop (admission by memory budget, continuous
+batching, shrinkwrap schedule, latency bookkeeping) against a simulated
+device backend with a virtual clock.  No GPU."""
+import importlib
+
+import numpy as np
+import pytest
+
+serve = importlib.import_module("nadia-coecxs_amd.serve")
+
+
+class Clock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += dt
+
+
+class FakeBackend:
+    """Each launch costs `cost` seconds per Mpx of the largest job in it;
+    jobs in one tick share launches, so a tick costs its largest count."""
+
+    def __init__(self, clock, cost=1e-3, bytes_per_px=100):
+        self.clock = clock
+        self.cost = cost
+        self.bpp = bytes_per_px
+        self.live = {}
+        self.used = 0
+        self.peak = 0
+        self.started = []
+        self.shrinks = {}
+        self.counts_seen = []
+
+    def bytes_for(self, req):
+        return req.n * req.n * self.bpp
+
+    def start(self, req):
+        self.used += self.bytes_for(req)
+        self.peak = max(self.peak, self.used)
+        h = {"req": req, "done": 0}
+        self.live[req.rid] = h
+        self.started.append(req.rid)
+        self.shrinks[req.rid] = []
+        return h
+
+    def run(self, handles, counts):
+        assert len(handles) == len(counts) and all(c >= 1 for c in counts)
+        self.counts_seen.append(list(counts))
+        for h, c in zip(handles, counts):
+            h["done"] += c
+        mpx = max(h["req"].n ** 2 for h in handles) / 1e6
+        self.clock.t += max(counts) * self.cost * mpx
+
+    def shrinkwrap(self, h, sigma, thr):
+        self.shrinks[h["req"].rid].append((h["done"], sigma, thr))
+
+    def wait(self):
+        pass
+
+    def finish(self, h, keep):
+        self.used -= self.bytes_for(h["req"])
+        del self.live[h["req"].rid]
+        return float(h["done"]), None
+
+
+def run(reqs, **kw):
+    clock = Clock()
+    be = FakeBackend(clock)
+    srv = serve.Server(be, **kw)
+    done = srv.run(reqs, clock=clock, sleep=clock.sleep)
+    return srv, be, done
+
+
+def test_stream_is_seeded_and_follows_the_distribution():
+    a = serve.poisson_stream(count=4000, seed=3)
+    b = serve.poisson_stream(count=4000, seed=3)
+    assert a == b
+    ns = np.array([r.n for r in a])
+    for n, p in zip((256, 512, 1024, 2048), (0.4, 0.3, 0.2, 0.1)):
+        assert abs((ns == n).mean() - p) < 0.03
+    its = np.array([r.iterations for r in a])
+    assert its.min() >= 100 and its.max() <= 1000
+    gaps = np.diff([0.0] + [r.arrival for r in a])
+    assert abs(gaps.mean() - 1 / 40.0) < 0.003
+    assert {(r.algorithm, r.beta) for r in a} == {(serve.ER, 0.9), (serve.HIO, 0.9), (serve.RAAR, 0.85)}
+
+
+def test_every_request_completes_with_its_iterations_and_schedule():
+    reqs = serve.poisson_stream(count=60, rate=200.0, sizes=(256, 512), probs=(0.5, 0.5), iterations=(40, 260),
+                                seed=5)
+    srv, be, done = run(reqs, hbm_budget=1e12)
+    assert sorted(c.rid for c in done) == [r.rid for r in reqs]
+    by = {r.rid: r for r in reqs}
+    for c in done:
+        r = by[c.rid]
+        assert c.iterations == r.iterations and c.error == r.iterations
+        assert c.finished >= c.admitted >= c.arrival - 1e-12
+        sw = be.shrinks[r.rid]
+        if r.shrinkwrap:
+            # every 50 iterations, never after the last one
+            assert [s[0] for s in sw] == list(range(50, r.iterations, 50))
+            sig = [s[1] for s in sw]
+            if len(sig) > 1:
+                assert sig[0] == 3.0 and abs(sig[-1] - 1.5) < 1e-12 and np.all(np.diff(sig) < 0)
+        else:
+            assert sw == []
+    # no tick gives a job more than `tick` iterations
+    assert max(max(c) for c in be.counts_seen) <= 50
+    assert srv.iterations == sum(r.iterations for r in reqs)
+
+
+def test_admission_respects_the_budget_in_arrival_order():
+    reqs = [serve.Request(i, 0.001 * i, 512 if i % 3 else 1024, serve.HIO, 0.9, 100 + 10 * i, False, i)
+            for i in range(30)]
+    budget = 4 * 1024 * 1024 * 100  # room for 4 jobs of 1024^2 (or 16 of 512^2)
+    srv, be, done = run(reqs, hbm_budget=budget)
+    assert be.peak <= budget and srv.peak_bytes <= budget
+    assert be.started == sorted(be.started)  # first come, first admitted
+    assert len(done) == 30
+    adm = {c.rid: c.admitted for c in done}
+    assert all(adm[i] <= adm[i + 1] for i in range(29))
+
+
+def test_continuous_batching_admits_while_others_run():
+    long_job = serve.Request(0, 0.0, 1024, serve.HIO, 0.9, 1000, False, 0)
+    short = [serve.Request(i, 0.05 * i, 256, serve.ER, 0.9, 100, False, i) for i in range(1, 6)]
+    srv, be, done = run([long_job] + short, hbm_budget=1e12)
+    fin = {c.rid: c for c in done}
+    for i in range(1, 6):
+        # admitted and finished while the long job is still running
+        assert fin[i].admitted < fin[0].finished
+        assert fin[i].finished < fin[0].finished
+    assert srv.peak_active >= 2
+
+
+def test_slot_limit_and_oversized_request():
+    reqs = [serve.Request(i, 0.0, 256, serve.ER, 0.9, 100, False, i) for i in range(8)]
+    srv, be, done = run(reqs, hbm_budget=1e12, max_active=3)
+    assert srv.peak_active == 3 and len(done) == 8
+    with pytest.raises(ValueError):
+        run([serve.Request(0, 0.0, 2048, serve.ER, 0.9, 100, False, 0)], hbm_budget=1e6)
+
+
+def test_summary_percentiles():
+    reqs = serve.poisson_stream(count=40, rate=100.0, sizes=(256,), probs=(1.0,), iterations=(100, 200), seed=9)
+    srv, be, done = run(reqs, hbm_budget=1e12)
+    s = serve.Server.summary(done, srv.elapsed)
+    lat = sorted(c.latency * 1e3 for c in done)
+    assert s["requests"] == 40
+    assert lat[0] <= s["p50_ms"] <= s["p99_ms"] <= lat[-1]
+    assert s["aggregate_iter_s"] > 0
