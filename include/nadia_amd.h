@@ -65,6 +65,10 @@ extern "C" {
 typedef struct nadia_field nadia_field; /* device-resident complex field */
 typedef struct nadia_recon nadia_recon; /* one reconstruction (BaseCDI object) */
 typedef struct nadia_batch nadia_batch; /* reconstructions iterated together */
+typedef struct nadia_constraint nadia_constraint; /* a TransmissionConstraint */
+/* host function applied to a transmission function (interleaved complex,
+ * x-major, nx x ny, float or double as `precision`), in place */
+typedef void (*nadia_custom_constraint_fn)(void* complex_host, int nx, int ny, int precision, void* user);
 
 const char* nadia_last_error(void);
 int nadia_version(void);
@@ -185,6 +189,38 @@ int nadia_wf_setup(nadia_recon* r, double wavelength, double zone_focal_length,
 /* set_support(z_diameter, size) (src/FresnelCDI_WF.c++:177-196) */
 int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size);
 
+/* ---- TransmissionConstraint (NADIA/include/TransmissionConstraint.h,
+ * NADIA/src/TransmissionConstraint.c++:26-172) ----
+ * create: unity magnitude and charge flipping (sign +1) enabled, like the
+ * reference's constructor.  add_region = add_complex_constraint with a
+ * ComplexConstraint(region, alpha1, alpha2): pixels with region > 0 (nx*ny
+ * doubles) join region `index`; a pixel already claimed keeps its first
+ * region.  set_fixed_c = ComplexConstraint::set_fixed_c; get_c_mean returns
+ * the value the last application computed (or the fixed one).  apply =
+ * apply_constraint on a field.  set_complex_constraint attaches one to a
+ * reconstruction (NULL detaches): apply_support then applies it after the
+ * support (to the estimate for PlanarCDI, to the transmission function for
+ * FresnelCDI) and iterate() takes the unfused path, as the constraint's
+ * per-region means need the whole field between projection and update.  The
+ * constraint is not owned and must outlive its use. */
+int nadia_constraint_create(int nx, int ny, nadia_constraint** out);
+int nadia_constraint_destroy(nadia_constraint* tc);
+int nadia_constraint_set_charge_flipping(nadia_constraint* tc, int enable, int flip_sign);
+int nadia_constraint_set_enforce_unity(nadia_constraint* tc, int enable);
+int nadia_constraint_add_region(nadia_constraint* tc, const double* region, double alpha1, double alpha2,
+                                int* index);
+/* ComplexConstraint::set_alpha1 / set_alpha2 of region `index` */
+int nadia_constraint_set_alphas(nadia_constraint* tc, int index, double alpha1, double alpha2);
+int nadia_constraint_set_fixed_c(nadia_constraint* tc, int index, double c_mean);
+int nadia_constraint_get_c_mean(nadia_constraint* tc, int index, double* c_mean);
+/* delete_complex_constraint_regions */
+int nadia_constraint_clear_regions(nadia_constraint* tc);
+/* set_custom_constraint: fn runs on the host after the built-in steps */
+int nadia_constraint_set_custom(nadia_constraint* tc, nadia_custom_constraint_fn fn, void* user);
+int nadia_constraint_apply(nadia_constraint* tc, nadia_field* f);
+/* BaseCDI::set_complex_constraint (NADIA/include/BaseCDI.h:471-473) */
+int nadia_recon_set_complex_constraint(nadia_recon* r, nadia_constraint* tc);
+
 /* Device memory (bytes) a reconstruction holds besides its estimate field,
  * and an estimate for one not yet created, estimate field included (fused
  * path, beam stop set): the admission figure of a request loop.  No
@@ -233,6 +269,8 @@ int nadia_batch_graph_replays(const nadia_batch* b, long long* n);
  * autotuner; get returns -1 before the first fused iterate(). */
 int nadia_recon_set_tile_variants(nadia_recon* r, int col_variant, int row_variant);
 int nadia_recon_get_tile_variants(const nadia_recon* r, int* col_variant, int* row_variant);
+/* number of column and row tile variants (valid variants are 0..count-1) */
+int nadia_tile_variant_count(int* col_variants, int* row_variants);
 
 
 /* Record the device time of every fused pass launched by iterate() with HIP

@@ -109,9 +109,22 @@ def _sig(lib):
         "nadia_batch_set_graph": (I, [P, I]),
         "nadia_batch_graph_replays": (I, [P, LP]),
         "nadia_recon_device_bytes": (I, [P, ctypes.POINTER(ctypes.c_size_t)]),
+        "nadia_constraint_create": (I, [I, I, PP]),
+        "nadia_constraint_destroy": (I, [P]),
+        "nadia_constraint_set_charge_flipping": (I, [P, I, I]),
+        "nadia_constraint_set_enforce_unity": (I, [P, I]),
+        "nadia_constraint_add_region": (I, [P, DP, D, D, IP]),
+        "nadia_constraint_set_alphas": (I, [P, I, D, D]),
+        "nadia_constraint_set_fixed_c": (I, [P, I, D]),
+        "nadia_constraint_get_c_mean": (I, [P, I, DP]),
+        "nadia_constraint_clear_regions": (I, [P]),
+        "nadia_constraint_set_custom": (I, [P, P, P]),
+        "nadia_constraint_apply": (I, [P, P]),
+        "nadia_recon_set_complex_constraint": (I, [P, P]),
         "nadia_recon_bytes_estimate": (I, [I, I, I, I, I, ctypes.POINTER(ctypes.c_size_t)]),
         "nadia_recon_set_tile_variants": (I, [P, I, I]),
         "nadia_recon_get_tile_variants": (I, [P, IP, IP]),
+        "nadia_tile_variant_count": (I, [IP, IP]),
         "nadia_recon_set_timing": (I, [P, I]),
         "nadia_recon_pass_times": (I, [P, DP, LP]),
     }
@@ -325,6 +338,12 @@ class BaseCDI:
     def propagate_from_detector(self, c):
         _check(library().nadia_recon_propagate_from_detector(self.h, c.h))
 
+    def set_complex_constraint(self, tc):
+        """BaseCDI::set_complex_constraint; None detaches.  The constraint is
+        kept alive by this object while attached."""
+        self._tc = tc
+        _check(library().nadia_recon_set_complex_constraint(self.h, None if tc is None else tc.h))
+
     def get_best_result(self, index=0):
         out = np.empty((self.nx, self.ny), self.cdt)
         err = ctypes.c_double()
@@ -440,6 +459,74 @@ class FresnelCDI_WF(BaseCDI):
         _check(library().nadia_wf_set_support(self.h, float(z_diameter), float(size)))
 
 
+CUSTOM_CONSTRAINT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p)
+
+
+class TransmissionConstraint:
+    """TransmissionConstraint (NADIA/include/TransmissionConstraint.h) over
+    nadia_constraint: unity magnitude and charge flipping on by default;
+    add_complex_constraint(region, alpha1, alpha2) adds a ComplexConstraint
+    region and returns its index."""
+
+    def __init__(self, nx, ny):
+        h = ctypes.c_void_p()
+        _check(library().nadia_constraint_create(int(nx), int(ny), ctypes.byref(h)))
+        self.h = h
+        self.nx, self.ny = nx, ny
+        self._custom = None
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.nadia_constraint_destroy(self.h)
+            self.h = None
+
+    def set_charge_flipping(self, enable, flip_sign=1):
+        _check(library().nadia_constraint_set_charge_flipping(self.h, int(bool(enable)), int(flip_sign)))
+
+    def set_enforce_unity(self, enable):
+        _check(library().nadia_constraint_set_enforce_unity(self.h, int(bool(enable))))
+
+    def add_complex_constraint(self, region, alpha1, alpha2, fixed_c=None):
+        r = np.ascontiguousarray(region, dtype=np.float64)
+        assert r.shape == (self.nx, self.ny)
+        k = ctypes.c_int()
+        _check(library().nadia_constraint_add_region(self.h, r.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                     float(alpha1), float(alpha2), ctypes.byref(k)))
+        if fixed_c is not None:
+            _check(library().nadia_constraint_set_fixed_c(self.h, k.value, float(fixed_c)))
+        return k.value
+
+    def set_alphas(self, index, alpha1, alpha2):
+        _check(library().nadia_constraint_set_alphas(self.h, int(index), float(alpha1), float(alpha2)))
+
+    def c_mean(self, index):
+        v = ctypes.c_double()
+        _check(library().nadia_constraint_get_c_mean(self.h, int(index), ctypes.byref(v)))
+        return v.value
+
+    def delete_complex_constraint_regions(self):
+        _check(library().nadia_constraint_clear_regions(self.h))
+
+    def set_custom_constraint(self, fn):
+        """fn(array) modifies the transmission function (a numpy view) in place."""
+        if fn is None:
+            self._custom = None
+            _check(library().nadia_constraint_set_custom(self.h, None, None))
+            return
+
+        def tramp(ptr, nx, ny, prec, user):
+            dt = np.complex128 if prec == FP64 else np.complex64
+            buf = (ctypes.c_char * (nx * ny * np.dtype(dt).itemsize)).from_address(ptr)
+            fn(np.frombuffer(buf, dtype=dt).reshape(nx, ny))
+
+        self._custom = CUSTOM_CONSTRAINT_FN(tramp)
+        _check(library().nadia_constraint_set_custom(self.h, ctypes.cast(self._custom, ctypes.c_void_p), None))
+
+    def apply_constraint(self, c):
+        _check(library().nadia_constraint_apply(self.h, c.h))
+
+
 class Batch:
     """Reconstructions iterated together (nadia_batch_*): jobs of the same
     precision, kind and size share every launch of the fused passes.  Each
@@ -497,6 +584,13 @@ class Batch:
         v = ctypes.c_longlong()
         _check(library().nadia_batch_graph_replays(self.h, ctypes.byref(v)))
         return v.value
+
+
+def tile_variant_count():
+    """(column variants, row variants) of the fused passes."""
+    c, r = ctypes.c_int(), ctypes.c_int()
+    _check(library().nadia_tile_variant_count(ctypes.byref(c), ctypes.byref(r)))
+    return c.value, r.value
 
 
 def bytes_estimate(nx, ny, double=False, kind=PLANAR, n_best=0):

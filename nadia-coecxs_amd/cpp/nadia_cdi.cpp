@@ -7,11 +7,16 @@
 #include <cmath>
 #include <cstdlib>
 #include <iostream>
+#include <cstdio>
+#include <string>
+#include <vector>
 
 #include "BaseCDI.h"
 #include "FresnelCDI.h"
 #include "FresnelCDI_WF.h"
 #include "PlanarCDI.h"
+#include "TransmissionConstraint.h"
+#include "io.h"
 
 using std::cout;
 using std::endl;
@@ -156,10 +161,140 @@ void BaseCDI::scale_intensity(Complex_2D& c) {
 
 void BaseCDI::set_fftw_type(int type) { complex.set_fftw_type(type); }
 
+// BaseCDI::set_complex_constraint (NADIA/include/BaseCDI.h:471-473): the
+// constraint is held by pointer and applied on the device by apply_support
 void BaseCDI::set_complex_constraint(TransmissionConstraint& trans_constraint) {
   transmission_constraint = &trans_constraint;
-  cout << "WARNING: complex (transmission) constraints are not applied by the device "
-       << "reconstruction; the constraint is ignored" << endl;
+  nadia_cxx_check(nadia_recon_set_complex_constraint(recon, trans_constraint.device_handle(nx, ny)),
+                  "set_complex_constraint");
+}
+
+// ---- TransmissionConstraint (NADIA/src/TransmissionConstraint.c++) ----------
+
+TransmissionConstraint::TransmissionConstraint()
+    : do_enforce_unity(true), do_charge_flip(true), flip_sign(1), custom_constraint(0), h(0), nx(0), ny(0) {}
+
+TransmissionConstraint::~TransmissionConstraint() {
+  if (h) nadia_constraint_destroy(h);
+}
+
+void TransmissionConstraint::push_settings() {
+  if (!h) return;
+  nadia_cxx_check(nadia_constraint_set_enforce_unity(h, do_enforce_unity ? 1 : 0), "set_enforce_unity");
+  nadia_cxx_check(nadia_constraint_set_charge_flipping(h, do_charge_flip ? 1 : 0, flip_sign), "set_charge_flipping");
+  nadia_cxx_check(nadia_constraint_set_custom(h, custom_constraint ? custom_trampoline : 0, this),
+                  "set_custom_constraint");
+}
+
+nadia_constraint* TransmissionConstraint::device_handle(int x, int y) {
+  if (!h) {
+    nadia_cxx_check(nadia_constraint_create(x, y, &h), "TransmissionConstraint");
+    nx = x;
+    ny = y;
+    push_settings();
+  } else if (x != nx || y != ny) {
+    cout << "Dimensions of the transmission function do not match the complex constraint regions" << endl;
+    exit(1);
+  }
+  return h;
+}
+
+void TransmissionConstraint::delete_complex_constraint_regions() {
+  while (!complex_constraint_list.empty()) {
+    delete complex_constraint_list.back();
+    complex_constraint_list.pop_back();
+  }
+  if (h) nadia_cxx_check(nadia_constraint_clear_regions(h), "delete_complex_constraint_regions");
+}
+
+// add_complex_constraint (NADIA/src/TransmissionConstraint.c++:43-77)
+void TransmissionConstraint::add_complex_constraint(ComplexConstraint& c) {
+  Double_2D* r = c.get_region();
+  const int rx = r->get_size_x(), ry = r->get_size_y();
+  if (h && (rx != nx || ry != ny)) {
+    cout << "Dimensions of the complex constraint region do not match the others given previously. "
+         << "Ignoring this complex constraint." << endl;
+    return;
+  }
+  device_handle(rx, ry);
+  std::vector<double> reg((size_t)rx * ry);
+  for (int i = 0; i < rx; i++)
+    for (int j = 0; j < ry; j++) reg[(size_t)i * ry + j] = r->get(i, j);
+  int k = -1;
+  nadia_cxx_check(nadia_constraint_add_region(h, reg.data(), c.alpha1, c.alpha2, &k), "add_complex_constraint");
+  if (c.fixed_c) nadia_cxx_check(nadia_constraint_set_fixed_c(h, k, c.c_mean), "set_fixed_c");
+  c.owner = this;
+  c.index = k;
+  complex_constraint_list.push_back(&c);
+}
+
+void TransmissionConstraint::set_charge_flipping(bool enable, int sign) {
+  do_charge_flip = enable;
+  flip_sign = sign > 0 ? 1 : -1;
+  push_settings();
+}
+
+void TransmissionConstraint::set_enforce_unity(bool enable) {
+  do_enforce_unity = enable;
+  push_settings();
+}
+
+void TransmissionConstraint::set_custom_constraint(void (*fn)(Complex_2D& transmission)) {
+  custom_constraint = fn;
+  push_settings();
+}
+
+// the custom constraint sees the transmission function as a Complex_2D
+void TransmissionConstraint::custom_trampoline(void* data, int x, int y, int precision, void* user) {
+  TransmissionConstraint* self = static_cast<TransmissionConstraint*>(user);
+  if (!self->custom_constraint) return;
+  Complex_2D t(x, y);
+  nadia_real* a = t.host_data_for_write();
+  const size_t n = 2 * (size_t)x * y;
+  if (precision == NADIA_FP64) {
+    const double* d = static_cast<const double*>(data);
+    for (size_t i = 0; i < n; i++) a[i] = (nadia_real)d[i];
+  } else {
+    const float* d = static_cast<const float*>(data);
+    for (size_t i = 0; i < n; i++) a[i] = (nadia_real)d[i];
+  }
+  self->custom_constraint(t);
+  const nadia_real* b = t.host_data();
+  if (precision == NADIA_FP64) {
+    double* d = static_cast<double*>(data);
+    for (size_t i = 0; i < n; i++) d[i] = (double)b[i];
+  } else {
+    float* d = static_cast<float*>(data);
+    for (size_t i = 0; i < n; i++) d[i] = (float)b[i];
+  }
+}
+
+// apply_constraint (NADIA/src/TransmissionConstraint.c++:81-172)
+void TransmissionConstraint::apply_constraint(Complex_2D& transmission) {
+  nadia_constraint* d = device_handle(transmission.get_size_x(), transmission.get_size_y());
+  nadia_cxx_check(nadia_constraint_apply(d, transmission.device_field()), "apply_constraint");
+  transmission.device_modified();
+}
+
+void ComplexConstraint::set_fixed_c(double c) {
+  fixed_c = true;
+  c_mean = c;
+  if (owner) nadia_cxx_check(nadia_constraint_set_fixed_c(owner->h, index, c), "set_fixed_c");
+}
+
+double ComplexConstraint::get_c_mean() {
+  if (owner && !fixed_c) nadia_cxx_check(nadia_constraint_get_c_mean(owner->h, index, &c_mean), "get_c_mean");
+  return c_mean;
+}
+
+void ComplexConstraint::set_alpha1(double a) {
+  alpha1 = a;
+  if (owner) nadia_cxx_check(nadia_constraint_set_alphas(owner->h, index, alpha1, alpha2), "set_alpha1");
+}
+
+void ComplexConstraint::set_alpha2(double a) {
+  alpha2 = a;
+  if (owner) nadia_cxx_check(nadia_constraint_set_alphas(owner->h, index, alpha1, alpha2), "set_alpha2");
 }
 
 void BaseCDI::reset_best() { nadia_cxx_check(nadia_recon_reset_best(recon), "reset_best"); }
@@ -334,4 +469,91 @@ void FresnelCDI_WF::multiply_factors(Complex_2D& c, int direction) {
       c.set_real(i, j, re * cr - im * ci);
       c.set_imag(i, j, im * cr + re * ci);
     }
+}
+
+// ---- .dbin / .cplx files (NADIA/src/io_dbin.c:24-100, NADIA/src/io_cplx.c:25-101)
+// Headerless doubles, y-major: element (x, y) at y*nx + x.
+
+int read_dbin(std::string file_name, int nx, int ny, Double_2D& data) {
+  FILE* file = fopen(file_name.c_str(), "rb");
+  if (!file) {
+    cout << "Could not open the file " << file_name << "." << endl;
+    return FAILURE;
+  }
+  std::vector<double> buffer((size_t)nx * ny);
+  const size_t got = fread(buffer.data(), sizeof(double), buffer.size(), file);
+  fclose(file);
+  if (got != buffer.size()) {
+    cout << "Could not correctly read the file" << file_name << ". Perhaps the dimensions specifies are wrong"
+         << " or the format type is not double?" << endl;
+    return FAILURE;
+  }
+  if (data.get_size_x() == 0) data.allocate_memory(nx, ny);
+  if (data.get_size_x() != nx || data.get_size_y() != ny) {
+    cout << "The Double_2D object supplied has the wrong dimensions" << endl;
+    return FAILURE;
+  }
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) data.set(i, j, buffer[(size_t)j * nx + i]);
+  return SUCCESS;
+}
+
+int write_dbin(std::string file_name, const Double_2D& data) {
+  FILE* file = fopen(file_name.c_str(), "w+b");
+  if (!file) {
+    cout << "Could not open the file " << file_name << endl;
+    return FAILURE;
+  }
+  const int nx = data.get_size_x(), ny = data.get_size_y();
+  std::vector<double> buffer((size_t)nx * ny);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) buffer[(size_t)j * nx + i] = data.get(i, j);
+  fwrite(buffer.data(), sizeof(double), buffer.size(), file);
+  fclose(file);
+  return SUCCESS;
+}
+
+int read_cplx(std::string file_name, Complex_2D& complex) {
+  const int nx = complex.get_size_x(), ny = complex.get_size_y();
+  FILE* file = fopen(file_name.c_str(), "rb");
+  if (!file) {
+    cout << "Could not open the file " << file_name << endl;
+    return FAILURE;
+  }
+  std::vector<double> buffer(2 * (size_t)nx * ny);
+  const size_t got = fread(buffer.data(), sizeof(double), buffer.size(), file);
+  fclose(file);
+  if (got != buffer.size()) {
+    cout << "Could not correctly read the file " << file_name << ". Perhaps the dimensions specifies are wrong?"
+         << endl;
+    return FAILURE;
+  }
+  nadia_real* a = complex.host_data_for_write();
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const size_t p = (size_t)j * nx + i, q = (size_t)i * ny + j;
+      a[2 * q] = (nadia_real)buffer[2 * p];
+      a[2 * q + 1] = (nadia_real)buffer[2 * p + 1];
+    }
+  return SUCCESS;
+}
+
+int write_cplx(std::string file_name, const Complex_2D& complex) {
+  const int nx = complex.get_size_x(), ny = complex.get_size_y();
+  FILE* file = fopen(file_name.c_str(), "w+b");
+  if (!file) {
+    cout << "Could not open the file " << file_name << endl;
+    return FAILURE;
+  }
+  std::vector<double> buffer(2 * (size_t)nx * ny);
+  const nadia_real* a = complex.host_data();
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const size_t p = (size_t)j * nx + i, q = (size_t)i * ny + j;
+      buffer[2 * p] = a[2 * q];
+      buffer[2 * p + 1] = a[2 * q + 1];
+    }
+  fwrite(buffer.data(), sizeof(double), buffer.size(), file);
+  fclose(file);
+  return SUCCESS;
 }

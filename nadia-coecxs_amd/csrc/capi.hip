@@ -170,6 +170,9 @@ struct nadia_recon {
   // fused-path caches
   bool w_valid, w2_valid;
   unsigned long long w_version, w2_version, w2_sversion, s_version;
+  // transmission constraint (not owned) and its Fresnel work field
+  nadia_constraint* tc;
+  void* ttmp;
   // Fresnel
   void *L, *Ls;
   double norm, wl, zfd, zfs, pixel, phase_factor;
@@ -192,6 +195,22 @@ struct nadia_recon {
   // [1] scratch copy used by the autotuner
   JobDesc* djob;
   double* tune_errs;
+};
+
+struct nadia_constraint {
+  int dev, nx, ny;
+  int nreg;
+  std::vector<unsigned char> hmap;  // region number per pixel (host copy)
+  std::vector<double> a1, a2, cm;   // per region; cm holds fixed values
+  std::vector<int> fixed;
+  int charge_flip, flip_sign, unity;
+  unsigned char* map;  // device
+  double* dpar;        // device: a1[MAXR], a2[MAXR], c_mean[MAXR]
+  int* dfixed;         // device
+  double* partials;    // device: kEwBlocks * MAXR * 2
+  bool dirty, map_dirty;
+  nadia_custom_constraint_fn custom;
+  void* user;
 };
 
 struct nadia_batch {
@@ -343,10 +362,27 @@ int op_project(nadia_recon* r, void* buf, double* slot) {
   return op_from_detector(r, buf);
 }
 
+int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec);
+
+// BaseCDI::apply_support (NADIA/src/BaseCDI.c++:130-134) and
+// FresnelCDI::apply_support (NADIA/src/FresnelCDI.c++:203-215): support,
+// then the transmission constraint on the estimate (planar) or on its
+// transmission function (Fresnel: get_transmission_function applies it,
+// set_transmission_function writes the exit-surface wave back).
 int op_support(nadia_recon* r, void* buf) {
   DISPATCH(r->prec, k_support<T>(buf, r->S, r->npix, S_(r)));
   LAUNCHCHK();
-  return 0;
+  if (!r->tc) return 0;
+  if (r->kind == NADIA_FRESNEL) {
+    int rc;
+    if (!r->ttmp && (rc = alloc(&r->ttmp, r->npix * r->esz))) return rc;
+    DISPATCH(r->prec, k_trans_from_esw<T>(buf, r->Ls, r->ttmp, r->npix, S_(r)));
+    if ((rc = tc_apply(r->tc, S_(r), r->ttmp, r->prec))) return rc;
+    DISPATCH(r->prec, k_esw_from_trans<T>(r->ttmp, r->Ls, buf, r->nx, r->ny, S_(r)));
+    LAUNCHCHK();
+    return 0;
+  }
+  return tc_apply(r->tc, S_(r), buf, r->prec);
 }
 
 int copy_field(nadia_recon* r, void* dst, const void* src) {
@@ -409,7 +445,9 @@ bool has_projection(const nadia_recon* r) {
   return a[A_PF] != 0 || a[A_PSF] != 0 || a[A_PFS] != 0;
 }
 
-bool fused_ok(const nadia_recon* r) { return r->path == NADIA_PATH_FUSED && has_projection(r); }
+// a transmission constraint needs whole-field reductions between the
+// projection and the update, so it runs the unfused path
+bool fused_ok(const nadia_recon* r) { return r->path == NADIA_PATH_FUSED && has_projection(r) && !r->tc; }
 
 struct Ev {
   hipEvent_t a, b;
@@ -1088,7 +1126,7 @@ int nadia_recon_destroy(nadia_recon* r) {
   if (!r) return 0;
   (void)hipStreamSynchronize(S_(r));
   void* ptrs[] = {r->S, r->Q, r->beam, r->W, r->t[0], r->t[1], r->t[2], r->t[3], r->conv, r->rtmp,
-                  r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs};
+                  r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs, r->ttmp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (void* p : r->best) (void)hipFree(p);
@@ -1557,6 +1595,12 @@ int nadia_fresnel_get_transmission_function(nadia_recon* r, const nadia_field* e
   int rc;
   if ((rc = alloc(&tmp, r->npix * r->esz))) return rc;
   DISPATCH(r->prec, k_trans_from_esw<T>(e->d, r->Ls, tmp, r->npix, S_(r)));
+  // the reference applies the transmission constraint here
+  // (NADIA/src/FresnelCDI.c++:349-352)
+  if (r->tc && (rc = tc_apply(r->tc, S_(r), tmp, r->prec))) {
+    (void)hipFree(tmp);
+    return rc;
+  }
   HIPCHK(hipMemcpyAsync(host, tmp, r->npix * r->esz, hipMemcpyDeviceToHost, S_(r)));
   HIPCHK(hipStreamSynchronize(S_(r)));
   (void)hipFree(tmp);
@@ -1629,6 +1673,221 @@ int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size) {
 // ---- measurement ---------------------------------------------------------
 
 // ---- batches ------------------------------------------------------------
+
+// ---- TransmissionConstraint -------------------------------------------
+
+}  // extern "C"
+
+namespace {
+constexpr int kMaxRegions = 255;  // region numbers are stored in one byte
+
+int tc_upload(nadia_constraint* tc, hipStream_t st) {
+  const size_t npix = (size_t)tc->nx * tc->ny;
+  if (tc->map_dirty && tc->nreg > 0) {
+    HIPCHK(hipMemcpyAsync(tc->map, tc->hmap.data(), npix, hipMemcpyHostToDevice, st));
+  }
+  if (tc->dirty) {
+    std::vector<double> par(3 * kMaxRegions, 0.0);
+    std::vector<int> fx(kMaxRegions, 0);
+    for (int k = 0; k < tc->nreg; ++k) {
+      par[k] = tc->a1[k];
+      par[kMaxRegions + k] = tc->a2[k];
+      par[2 * kMaxRegions + k] = tc->cm[k];
+      fx[k] = tc->fixed[k];
+    }
+    HIPCHK(hipMemcpyAsync(tc->dpar, par.data(), par.size() * sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(tc->dfixed, fx.data(), fx.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // par / fx live on this stack frame
+  } else if (tc->map_dirty) {
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  tc->dirty = tc->map_dirty = false;
+  return 0;
+}
+
+TcDev tc_dev(const nadia_constraint* tc) {
+  TcDev d{};
+  d.map = tc->map;
+  d.nregions = tc->nreg;
+  d.alpha1 = tc->dpar;
+  d.alpha2 = tc->dpar + kMaxRegions;
+  d.c_mean = tc->dpar + 2 * kMaxRegions;
+  d.fixed = tc->dfixed;
+  d.charge_flip = tc->charge_flip;
+  d.flip_sign = tc->flip_sign;
+  d.unity = tc->unity;
+  return d;
+}
+
+// TransmissionConstraint::apply_constraint (NADIA/src/TransmissionConstraint.c++:81-172)
+int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec) {
+  int rc;
+  if ((rc = tc_upload(tc, st))) return rc;
+  const long long n = (long long)tc->nx * tc->ny;
+  const TcDev d = tc_dev(tc);
+  if (tc->nreg > 0) {
+    DISPATCH(prec, k_tc_stats<T>(buf, d, n, tc->partials, st));
+    k_tc_final(d, tc->partials, kEwBlocks, st);
+    LAUNCHCHK();
+  }
+  if (tc->nreg > 0 || tc->charge_flip || tc->unity) {
+    DISPATCH(prec, k_tc_apply<T>(buf, d, n, st));
+    LAUNCHCHK();
+  }
+  if (tc->custom) {
+    // the custom constraint is a host function: hand it the array
+    const size_t bytes = (size_t)n * (prec == NADIA_FP64 ? 16 : 8);
+    std::vector<char> h(bytes);
+    HIPCHK(hipMemcpyAsync(h.data(), buf, bytes, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    tc->custom(h.data(), tc->nx, tc->ny, prec, tc->user);
+    HIPCHK(hipMemcpyAsync(buf, h.data(), bytes, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int nadia_constraint_create(int nx, int ny, nadia_constraint** out) {
+  if (!out || nx <= 0 || ny <= 0) return fail(NADIA_EINVAL, "bad argument");
+  nadia_constraint* tc = new nadia_constraint{};
+  tc->dev = current_device();
+  tc->nx = nx;
+  tc->ny = ny;
+  // TransmissionConstraint::TransmissionConstraint: unity and charge flipping on
+  tc->unity = 1;
+  tc->charge_flip = 1;
+  tc->flip_sign = 1;
+  tc->dirty = true;
+  int rc;
+  if ((rc = alloc((void**)&tc->map, (size_t)nx * ny)) || (rc = alloc((void**)&tc->dpar, 3 * kMaxRegions * sizeof(double))) ||
+      (rc = alloc((void**)&tc->dfixed, kMaxRegions * sizeof(int))) ||
+      (rc = alloc((void**)&tc->partials, (size_t)kEwBlocks * kMaxRegions * 2 * sizeof(double)))) {
+    nadia_constraint_destroy(tc);
+    return rc;
+  }
+  *out = tc;
+  return 0;
+}
+
+int nadia_constraint_destroy(nadia_constraint* tc) {
+  if (!tc) return 0;
+  DeviceCtx* c = dev_ctx(tc->dev);
+  if (c) (void)hipStreamSynchronize(c->stream);
+  void* ptrs[] = {tc->map, tc->dpar, tc->dfixed, tc->partials};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete tc;
+  return 0;
+}
+
+int nadia_constraint_set_charge_flipping(nadia_constraint* tc, int enable, int flip_sign) {
+  if (!tc) return fail(NADIA_EINVAL, "null constraint");
+  tc->charge_flip = enable ? 1 : 0;
+  tc->flip_sign = flip_sign > 0 ? 1 : -1;
+  return 0;
+}
+
+int nadia_constraint_set_enforce_unity(nadia_constraint* tc, int enable) {
+  if (!tc) return fail(NADIA_EINVAL, "null constraint");
+  tc->unity = enable ? 1 : 0;
+  return 0;
+}
+
+int nadia_constraint_add_region(nadia_constraint* tc, const double* region, double alpha1, double alpha2,
+                                int* index) {
+  if (!tc || !region) return fail(NADIA_EINVAL, "null argument");
+  if (tc->nreg >= kMaxRegions) return fail(NADIA_EINVAL, "at most %d complex-constraint regions", kMaxRegions);
+  const size_t npix = (size_t)tc->nx * tc->ny;
+  if (tc->hmap.empty()) tc->hmap.assign(npix, 0);
+  const int k = tc->nreg++;
+  bool overlap = false;
+  for (size_t p = 0; p < npix; ++p)
+    if (region[p] > 0) {
+      if (tc->hmap[p] > 0) overlap = true;  // the earlier region keeps the pixel
+      else tc->hmap[p] = (unsigned char)(k + 1);
+    }
+  if (overlap)
+    fprintf(stdout, "Overlap between complex constraint region detected. The last constraint to be set will be "
+                    "ignored in the overlapping region.\n");
+  tc->a1.push_back(alpha1);
+  tc->a2.push_back(alpha2);
+  tc->cm.push_back(0.0);
+  tc->fixed.push_back(0);
+  tc->dirty = tc->map_dirty = true;
+  if (index) *index = k;
+  return 0;
+}
+
+int nadia_constraint_set_alphas(nadia_constraint* tc, int index, double alpha1, double alpha2) {
+  if (!tc || index < 0 || index >= tc->nreg) return fail(NADIA_EINVAL, "bad region index");
+  tc->a1[index] = alpha1;
+  tc->a2[index] = alpha2;
+  tc->dirty = true;
+  return 0;
+}
+
+int nadia_constraint_set_fixed_c(nadia_constraint* tc, int index, double c_mean) {
+  if (!tc || index < 0 || index >= tc->nreg) return fail(NADIA_EINVAL, "bad region index");
+  tc->fixed[index] = 1;
+  tc->cm[index] = c_mean;
+  tc->dirty = true;
+  return 0;
+}
+
+int nadia_constraint_get_c_mean(nadia_constraint* tc, int index, double* c_mean) {
+  if (!tc || !c_mean || index < 0 || index >= tc->nreg) return fail(NADIA_EINVAL, "bad region index");
+  DeviceCtx* c = dev_ctx(tc->dev);
+  if (!c) return fail(NADIA_EHIP, "stream creation failed");
+  if (tc->dirty) {
+    *c_mean = tc->cm[index];
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(c_mean, tc->dpar + 2 * kMaxRegions + index, sizeof(double), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int nadia_constraint_clear_regions(nadia_constraint* tc) {
+  if (!tc) return fail(NADIA_EINVAL, "null constraint");
+  tc->nreg = 0;
+  tc->hmap.clear();
+  tc->a1.clear();
+  tc->a2.clear();
+  tc->cm.clear();
+  tc->fixed.clear();
+  tc->dirty = true;
+  return 0;
+}
+
+int nadia_constraint_set_custom(nadia_constraint* tc, nadia_custom_constraint_fn fn, void* user) {
+  if (!tc) return fail(NADIA_EINVAL, "null constraint");
+  tc->custom = fn;
+  tc->user = user;
+  return 0;
+}
+
+int nadia_constraint_apply(nadia_constraint* tc, nadia_field* f) {
+  if (!tc || !f) return fail(NADIA_EINVAL, "null argument");
+  if (f->nx != tc->nx || f->ny != tc->ny) return fail(NADIA_ESIZE, "field and constraint sizes differ");
+  DeviceCtx* c = dev_ctx(f->dev);
+  if (!c) return fail(NADIA_EHIP, "stream creation failed");
+  int rc = tc_apply(tc, c->stream, f->d, f->prec);
+  f->version++;
+  return rc;
+}
+
+int nadia_recon_set_complex_constraint(nadia_recon* r, nadia_constraint* tc) {
+  if (!r) return fail(NADIA_EINVAL, "null recon");
+  if (tc && (tc->nx != r->nx || tc->ny != r->ny)) return fail(NADIA_ESIZE, "constraint and reconstruction sizes differ");
+  if (tc && tc->dev != r->dev) return fail(NADIA_EINVAL, "constraint lives on another device");
+  r->tc = tc;
+  r->w_valid = r->w2_valid = false;
+  return 0;
+}
 
 int nadia_batch_create(nadia_batch** out) {
   if (!out) return fail(NADIA_EINVAL, "null output pointer");
@@ -1921,6 +2180,13 @@ int nadia_batch_iterate_each(nadia_batch* b, const int* counts) {
   if (!b || (!counts && !b->jobs.empty())) return fail(NADIA_EINVAL, "null argument");
   if (b->jobs.empty()) return 0;
   return batch_run(b, std::vector<int>(counts, counts + b->jobs.size()), nullptr);
+}
+
+int nadia_tile_variant_count(int* col_variants, int* row_variants) {
+  if (!col_variants || !row_variants) return fail(NADIA_EINVAL, "null argument");
+  *col_variants = kColVariants;
+  *row_variants = kRowVariants;
+  return 0;
 }
 
 int nadia_recon_set_tile_variants(nadia_recon* r, int col_variant, int row_variant) {

@@ -112,14 +112,40 @@ __device__ __forceinline__ void dft(c2_t<T> (&a)[R]) {
   }
 }
 
+// Twiddles of stage S for the NB = P/R butterflies of one thread:
+// w[q][m-1] = W_{L R}^{k m}, k = (t + q*TPT) mod L (sign -1 values).
+template <typename T, int N, int P, int S>
+struct StageTw {
+  static constexpr PlanDesc pd = make_plan(N, P);
+  static constexpr int R = (S < pd.nst) ? pd.r[S] : 2;
+  c2_t<T> w[P / R][R - 1];
+};
+
+template <typename T, int N, int P, int S>
+__device__ __forceinline__ void load_tw(StageTw<T, N, P, S>& w, int t, const c2_t<T>* __restrict__ tw) {
+  constexpr PlanDesc pd = make_plan(N, P);
+  if constexpr (S > 0 && S < pd.nst) {
+    constexpr int R = pd.r[S], L = pd.L[S], TPT = N / P, NB = P / R;
+#pragma unroll
+    for (int q = 0; q < NB; ++q) {
+      const int k = (t + q * TPT) % L;
+      const c2_t<T>* p = tw + pd.tw_off[S] + k * (R - 1);
+#pragma unroll
+      for (int m = 1; m < R; ++m) w.w[q][m - 1] = p[m - 1];
+    }
+  }
+}
+
 // One transform of length N held by TPT = N/P threads, P points each.
 // `lds` addresses position pos of this transform at lds[pos * lstride]
 // (the caller offsets the pointer per transform); `tw` is the plan's twiddle
-// table (sign -1 values; DIR > 0 uses their conjugates).  Every thread of the
-// workgroup must call this (it contains barriers).
-template <typename T, int N, int P, int DIR, int S = 0>
-__device__ __forceinline__ void fft_stages(c2_t<T> (&v)[P], c2_t<T>* lds, int lstride, int t,
-                                           const c2_t<T>* __restrict__ tw) {
+// table (sign -1 values; DIR > 0 uses their conjugates).  The twiddles of
+// stage S+1 are loaded before stage S's exchange barrier, so their latency
+// overlaps the exchange.  Every thread of the workgroup must call this (it
+// contains barriers).
+template <typename T, int N, int P, int DIR, int S>
+__device__ __forceinline__ void fft_stage(c2_t<T> (&v)[P], c2_t<T>* lds, int lstride, int t,
+                                          const c2_t<T>* __restrict__ tw, const StageTw<T, N, P, S>& ws) {
   using C = c2_t<T>;
   constexpr PlanDesc pd = make_plan(N, P);
   static_assert(pd.nst > 0, "unsupported FFT length");
@@ -136,11 +162,9 @@ __device__ __forceinline__ void fft_stages(c2_t<T> (&v)[P], c2_t<T>* lds, int ls
 #pragma unroll
       for (int m = 0; m < R; ++m) a[m] = v[q + m * NB];
       if constexpr (S > 0) {
-        const int k = j % L;
-        const C* w = tw + pd.tw_off[S] + k * (R - 1);
 #pragma unroll
         for (int m = 1; m < R; ++m) {
-          C wm = w[m - 1];
+          C wm = ws.w[q][m - 1];
           if (DIR > 0) wm.y = -wm.y;
           a[m] = cmul(a[m], wm);
         }
@@ -157,13 +181,22 @@ __device__ __forceinline__ void fft_stages(c2_t<T> (&v)[P], c2_t<T>* lds, int ls
       }
     }
     if constexpr (!last) {
+      StageTw<T, N, P, S + 1> wn;
+      load_tw<T, N, P, S + 1>(wn, t, tw);
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < P; ++i) v[i] = lds[(t + i * TPT) * lstride];
       __syncthreads();
-      fft_stages<T, N, P, DIR, S + 1>(v, lds, lstride, t, tw);
+      fft_stage<T, N, P, DIR, S + 1>(v, lds, lstride, t, tw, wn);
     }
   }
+}
+
+template <typename T, int N, int P, int DIR>
+__device__ __forceinline__ void fft_stages(c2_t<T> (&v)[P], c2_t<T>* lds, int lstride, int t,
+                                           const c2_t<T>* __restrict__ tw) {
+  StageTw<T, N, P, 0> w0;
+  fft_stage<T, N, P, DIR, 0>(v, lds, lstride, t, tw, w0);
 }
 
 // Wavefront + workgroup sum of two doubles; result valid in thread 0.

@@ -51,6 +51,24 @@ __global__ void __launch_bounds__(256) final_jobs_kernel(const JobDesc* jobs, in
   if (threadIdx.x == 0) j->errs[iter] = a / b;
 }
 
+// one workgroup per region: c_mean = sum |log mag| / sum |phase| over the
+// region (ComplexConstraint::set_c_mean leaves a fixed value alone)
+__global__ void __launch_bounds__(256) tc_final_kernel(TcDev d, const double* p, int nblocks) {
+  __shared__ double red[16];
+  const int k = blockIdx.x;
+  double a = 0, b = 0;
+  for (int i = threadIdx.x; i < nblocks; i += blockDim.x) {
+    a += p[2 * ((long long)i * d.nregions + k)];
+    b += p[2 * ((long long)i * d.nregions + k) + 1];
+  }
+  block_sum2(a, b, red);
+  if (threadIdx.x == 0 && !d.fixed[k]) d.c_mean[k] = a / b;
+}
+
+void k_tc_final(const TcDev& d, const double* partials, int nblocks, hipStream_t st) {
+  if (d.nregions > 0) hipLaunchKernelGGL(tc_final_kernel, dim3(d.nregions), dim3(256), 0, st, d, partials, nblocks);
+}
+
 void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st) {
   hipLaunchKernelGGL(final_jobs_kernel, dim3(njobs), dim3(256), 0, st, jobs, n, iter);
 }

@@ -88,6 +88,17 @@ template <typename T> void launch_col(const ColArgs& a, int njobs, int v, hipStr
 template <typename T> int col_tile(int nx, int v);
 template <typename T> int row_blocks(int nx, int ny, int v);
 
+// TransmissionConstraint state on the device (NADIA/include/TransmissionConstraint.h)
+struct TcDev {
+  const unsigned char* map;  // region number 1..nregions per pixel, 0 outside
+  int nregions;
+  const double* alpha1;      // per region
+  const double* alpha2;
+  double* c_mean;            // per region; recomputed unless fixed
+  const int* fixed;
+  int charge_flip, flip_sign, unity;
+};
+
 // element-wise and reduction kernels (kern_misc.inc)
 template <typename T> void k_invert(void* buf, int nx, int ny, int scale, hipStream_t st);
 template <typename T> void k_scale_intensity(void* buf, const void* q, int nx, int ny, double* partials, int nblocks, hipStream_t st);
@@ -112,6 +123,11 @@ template <typename T> void k_threshold(const void* conv, void* out, int nx, int 
                                        hipStream_t st);
 template <typename T> void k_normalise(const void* src, void* dst, const double* maxv, long long n, hipStream_t st);
 template <typename T> void k_wf_disk(void* s, int nx, int ny, double centre, double dz, double radius, hipStream_t st);
+// TransmissionConstraint::apply_constraint: per-region sums (partials of
+// kEwBlocks workgroups), their reduction to c_mean, and the per-pixel update
+template <typename T> void k_tc_stats(const void* t, const TcDev& d, long long n, double* partials, hipStream_t st);
+void k_tc_final(const TcDev& d, const double* partials, int nblocks, hipStream_t st);
+template <typename T> void k_tc_apply(void* t, const TcDev& d, long long n, hipStream_t st);
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st);
 // per-job final error reduction of the white-field row pass (grid = njobs)
 void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st);

@@ -90,7 +90,7 @@ constexpr int col_batch(int n, int esz) {
 // workgroup and in the occupancy target given to the register allocator.
 struct TileCfg {
   int p;        // points per thread
-  int batch;    // transforms (columns or rows) per workgroup
+  int batch;    // transforms (columns or rows) per workgroup at a time
   int threads;  // workgroup size
   int minw;     // waves per SIMD the register allocator must allow
 };
@@ -101,6 +101,8 @@ constexpr int kRowVariants = 2;
 // lengths that get more than the default variant
 constexpr bool tuned_size(int n) { return n == 512 || n == 1024 || n == 2048 || n == 4096; }
 
+// Column variants: 0 default; 1 registers limited so that two workgroups
+// share a CU; 2 half the columns per workgroup.
 constexpr TileCfg col_cfg(int n, int esz, int v) {
   TileCfg c{col_points(n), col_batch(n, esz), 0, 1};
   if (!tuned_size(n)) v = 0;
@@ -113,10 +115,13 @@ constexpr TileCfg col_cfg(int n, int esz, int v) {
   return c;
 }
 
-constexpr TileCfg row_cfg(int n, int v) {
+// Row variants: 0 default; 1 the same workgroup with registers limited so
+// that 8 waves (fp32; 4 in fp64) share a SIMD.  esz: bytes per complex
+// element.
+constexpr TileCfg row_cfg(int n, int v, int esz = 8) {
   TileCfg c{row_points(n), row_batch(n), 0, 1};
   if (!tuned_size(n)) v = 0;
-  if (v == 1) c.batch = c.batch * 2;  // two rows' worth of threads per workgroup
+  if (v == 1) c.minw = esz > 8 ? 4 : 8;
   c.threads = c.batch * (n / c.p);
   return c;
 }

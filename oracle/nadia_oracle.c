@@ -266,6 +266,8 @@ typedef struct {
   double norm, wavelength, pixel, zfd, zfs;
   /* FresnelCDI_WF */
   double z12, z23;
+  /* transmission constraint (not owned), NULL when none is set */
+  struct otc *tc;
 } ocdi;
 
 #define NPIX(o) ((size_t)(o)->nx * (o)->ny)
@@ -358,7 +360,102 @@ static void support_constraint(ocdi *o, cplx *c) {
   }
 }
 
-/* ---- FresnelCDI transmission helpers (no TransmissionConstraint) ------- */
+/* ---- TransmissionConstraint ------------------------------------------- */
+/* NADIA/src/TransmissionConstraint.c++:26-172 with ComplexConstraint from
+ * NADIA/include/TransmissionConstraint.h.  Regions are numbered 1..n in
+ * region_map (the first region to claim a pixel keeps it); the custom
+ * constraint callback is not modelled. */
+#define OTC_MAX 32
+typedef struct otc {
+  int nx, ny, nregions;
+  real *region_map;
+  double alpha1[OTC_MAX], alpha2[OTC_MAX], c_mean[OTC_MAX];
+  int fixed_c[OTC_MAX];
+  int enforce_unity, charge_flip, flip_sign;
+} otc;
+
+/* TransmissionConstraint::TransmissionConstraint: unity and charge flipping
+ * (sign +1) enabled */
+otc *otc_create(int nx, int ny) {
+  otc *t = (otc *)calloc(1, sizeof(otc));
+  t->nx = nx; t->ny = ny;
+  t->enforce_unity = 1; t->charge_flip = 1; t->flip_sign = 1;
+  return t;
+}
+
+void otc_destroy(otc *t) { if (t) { free(t->region_map); free(t); } }
+
+void otc_set_charge_flipping(otc *t, int enable, int flip_sign) {
+  t->charge_flip = enable;
+  t->flip_sign = flip_sign > 0 ? 1 : -1;
+}
+
+void otc_set_enforce_unity(otc *t, int enable) { t->enforce_unity = enable; }
+
+/* add_complex_constraint: NADIA/src/TransmissionConstraint.c++:43-77;
+ * returns the region index or -1 */
+int otc_add_region(otc *t, const real *region, double alpha1, double alpha2) {
+  if (t->nregions >= OTC_MAX) return -1;
+  if (!t->region_map) t->region_map = calloc_r((size_t)t->nx * t->ny);
+  int k = t->nregions++;
+  t->alpha1[k] = alpha1; t->alpha2[k] = alpha2; t->fixed_c[k] = 0; t->c_mean[k] = 0;
+  for (size_t p = 0; p < (size_t)t->nx * t->ny; p++)
+    if (region[p] > 0 && !(t->region_map[p] > 0)) t->region_map[p] = (real)(k + 1);
+  return k;
+}
+
+/* ComplexConstraint::set_fixed_c */
+void otc_set_fixed_c(otc *t, int k, double c) { t->fixed_c[k] = 1; t->c_mean[k] = c; }
+double otc_get_c_mean(otc *t, int k) { return t->c_mean[k]; }
+
+static inline double c_phase(cplx c) { return (real)atan2(c.im, c.re); }
+
+/* Complex_2D::set_mag / set_phase (NADIA/include/Complex_2D.h:205-230) */
+static inline void c_set_mag_d(cplx *c, double value) {
+  double mag = c_mag(*c);
+  if (mag == 0) { c->re = (real)value; c->im = 0; }
+  else { c->re = (real)(c->re * (value / mag)); c->im = (real)(c->im * (value / mag)); }
+}
+static inline void c_set_phase_d(cplx *c, double value) {
+  double mag = c_mag(*c);
+  c->re = (real)(mag * cos(value));
+  c->im = (real)(mag * sin(value));
+}
+
+/* apply_constraint: NADIA/src/TransmissionConstraint.c++:81-172 */
+void otc_apply(otc *t, cplx *tr) {
+  size_t n = (size_t)t->nx * t->ny;
+  int R = t->nregions;
+  if (R > 0) {
+    double lnA[OTC_MAX] = {0}, ph[OTC_MAX] = {0};
+    for (size_t p = 0; p < n; p++)
+      if (t->region_map[p] > 0) {
+        int k = (int)t->region_map[p] - 1;
+        ph[k] += fabs((double)(real)c_phase(tr[p]));
+        lnA[k] += fabs(log((double)c_mag(tr[p])));
+      }
+    for (int k = 0; k < R; k++)
+      if (!t->fixed_c[k]) t->c_mean[k] = lnA[k] / ph[k];
+  }
+  if (!(R > 0 || t->charge_flip || t->enforce_unity)) return;
+  for (size_t p = 0; p < n; p++) {
+    double phase_old = (real)c_phase(tr[p]);
+    double mag_old = c_mag(tr[p]);
+    if (R > 0 && t->region_map[p] > 0) {
+      int k = (int)t->region_map[p] - 1;
+      double c = t->c_mean[k];
+      double mag_new = exp((1 - t->alpha1[k]) * log(mag_old) + t->alpha1[k] * c * phase_old);
+      double phase_new = (1 - t->alpha2[k]) * phase_old + t->alpha2[k] * log(mag_old) / c;
+      c_set_mag_d(&tr[p], mag_new);
+      c_set_phase_d(&tr[p], phase_new);
+      mag_old = mag_new;
+    }
+    if (t->charge_flip && t->flip_sign * tr[p].im > 0) tr[p].im = -tr[p].im;
+    if (t->enforce_unity && mag_old > 1) c_set_mag_d(&tr[p], 1.0);
+  }
+}
+
+/* ---- FresnelCDI transmission helpers ------------------------------------ */
 
 /* FresnelCDI::set_transmission_function: NADIA/src/FresnelCDI.c++:253-283
  * (the loop runs j < nx: square arrays only, as in the reference) */
@@ -389,6 +486,7 @@ void ocdi_get_transmission_function(ocdi *o, cplx *result, const cplx *esw) {
       result[p].im = (real)(in / den);
     } else { result[p].re = 1.0; result[p].im = 0.0; }
   }
+  if (o->tc) otc_apply(o->tc, result);
 }
 
 /* ---- virtual methods dispatched on kind ----------------------------- */
@@ -482,11 +580,21 @@ void ocdi_scale_intensity(ocdi *o, cplx *c) {
   }
 }
 
+void ocdi_set_complex_constraint(ocdi *o, otc *t) { o->tc = t; }
+
 /* BaseCDI::apply_support: NADIA/src/BaseCDI.c++:130-134;
- * FresnelCDI::apply_support: NADIA/src/FresnelCDI.c++:203-215 (the
- * transmission round trip only runs with a TransmissionConstraint, which this
- * oracle does not model) */
-void ocdi_apply_support(ocdi *o, cplx *c) { support_constraint(o, c); }
+ * FresnelCDI::apply_support: NADIA/src/FresnelCDI.c++:203-215 */
+void ocdi_apply_support(ocdi *o, cplx *c) {
+  support_constraint(o, c);
+  if (!o->tc) return;
+  if (o->kind == KIND_FRESNEL) {
+    if (!o->transmission) o->transmission = calloc_c(NPIX(o));
+    ocdi_get_transmission_function(o, o->transmission, c);  /* applies the constraint */
+    ocdi_set_transmission_function(o, o->transmission, c);
+  } else {
+    otc_apply(o->tc, c);
+  }
+}
 
 /* BaseCDI::project_intensity: NADIA/src/BaseCDI.c++:156-160 */
 void ocdi_project_intensity(ocdi *o, cplx *c) {

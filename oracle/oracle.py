@@ -72,6 +72,15 @@ def _lib(double=True):
         "ocdi_wf_init": (None, [P, D, D, D, D]),
         "ocdi_wf_set_support": (None, [P, D, D]),
         "oracle_real_size": (I, []),
+        "otc_create": (P, [I, I]),
+        "otc_destroy": (None, [P]),
+        "otc_set_charge_flipping": (None, [P, I, I]),
+        "otc_set_enforce_unity": (None, [P, I]),
+        "otc_add_region": (I, [P, P, D, D]),
+        "otc_set_fixed_c": (None, [P, I, D]),
+        "otc_get_c_mean": (D, [P, I]),
+        "otc_apply": (None, [P, P]),
+        "ocdi_set_complex_constraint": (None, [P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -98,6 +107,44 @@ def invert(a, scale=True, double=True):
     lib = _lib(double)
     lib.or_invert(_ptr(a), a.shape[0], a.shape[1], 1 if scale else 0)
     return a
+
+
+class OracleConstraint:
+    """TransmissionConstraint (NADIA/src/TransmissionConstraint.c++:26-172)."""
+
+    def __init__(self, nx, ny, double=True):
+        self.double = double
+        self.lib = _lib(double)
+        self.rdt = np.float64 if double else np.float32
+        self.cdt = np.complex128 if double else np.complex64
+        self.h = self.lib.otc_create(nx, ny)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.otc_destroy(self.h)
+            self.h = None
+
+    def set_charge_flipping(self, enable, flip_sign=1):
+        self.lib.otc_set_charge_flipping(self.h, int(enable), int(flip_sign))
+
+    def set_enforce_unity(self, enable):
+        self.lib.otc_set_enforce_unity(self.h, int(enable))
+
+    def add_complex_constraint(self, region, alpha1, alpha2, fixed_c=None):
+        r = np.ascontiguousarray(region, dtype=self.rdt)
+        k = self.lib.otc_add_region(self.h, _ptr(r), alpha1, alpha2)
+        if fixed_c is not None:
+            self.lib.otc_set_fixed_c(self.h, k, fixed_c)
+        return k
+
+    def c_mean(self, k):
+        return self.lib.otc_get_c_mean(self.h, k)
+
+    def apply(self, t):
+        """apply_constraint on a complex array, in place."""
+        assert t.dtype == self.cdt
+        self.lib.otc_apply(self.h, _ptr(t))
+        return t
 
 
 class OracleCDI:
@@ -201,6 +248,10 @@ class OracleCDI:
 
     def propagate_from_detector(self, c):
         self.lib.ocdi_propagate_from_detector(self.h, _ptr(c))
+
+    def set_complex_constraint(self, tc):
+        self.tc = tc  # keep it alive: the object holds a non-owning pointer
+        self.lib.ocdi_set_complex_constraint(self.h, None if tc is None else tc.h)
 
     def get_best_result(self, index=0):
         out = np.zeros((self.nx, self.ny), self.cdt)

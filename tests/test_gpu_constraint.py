@@ -1,0 +1,124 @@
+"""GPU parity of TransmissionConstraint (nadia_constraint) against the CPU
+oracle: the constraint alone on a field, and inside apply_support during
+PlanarCDI and FresnelCDI iterations (which then take the unfused path)."""
+import numpy as np
+import pytest
+
+from test_gpu_fresnel import fresnel_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def field(n, seed):
+    rng = np.random.default_rng(seed)
+    mag = rng.uniform(0.2, 1.4, (n, n))
+    return mag * np.exp(1j * rng.uniform(-np.pi, np.pi, (n, n)))
+
+
+def regions(n):
+    a = np.zeros((n, n))
+    a[n // 8:n // 2, n // 8:5 * n // 8] = 1
+    b = np.zeros((n, n))
+    b[3 * n // 8:7 * n // 8, n // 4:3 * n // 4] = 1  # overlaps a: a keeps the shared pixels
+    return a, b
+
+
+def make_pair(nd, oracle, n, double, flip=(True, 1), unity=True, fixed=None, regs=None):
+    g = nd.TransmissionConstraint(n, n)
+    o = oracle.OracleConstraint(n, n, double=double)
+    a, b = regs if regs is not None else regions(n)
+    for tc in (g, o):
+        tc.set_charge_flipping(*flip)
+        tc.set_enforce_unity(unity)
+        tc.add_complex_constraint(a, 0.6, 0.3)
+        tc.add_complex_constraint(b, 0.2, 0.9, fixed_c=fixed)
+    return g, o
+
+
+@pytest.mark.parametrize("double,tol", [(True, 1e-12), (False, 1e-5)], ids=["fp64", "fp32"])
+@pytest.mark.parametrize("flip,unity,fixed", [((True, 1), True, None), ((True, -1), False, 0.7),
+                                              ((False, 1), True, None)])
+def test_constraint_matches_oracle(nd, oracle, double, tol, flip, unity, fixed):
+    n = 96
+    cdt = np.complex128 if double else np.complex64
+    t = field(n, 3).astype(cdt)
+    g, o = make_pair(nd, oracle, n, double, flip, unity, fixed)
+    f = nd.Complex2D(n, n, double=double, data=t)
+    g.apply_constraint(f)
+    to = t.copy()
+    o.apply(to)
+    assert rel(f.get(), to) <= tol
+    for k in (0, 1):
+        assert abs(g.c_mean(k) - o.c_mean(k)) <= tol * abs(o.c_mean(k))
+
+
+def test_custom_constraint_runs_after_builtin(nd, oracle):
+    n = 64
+    t = field(n, 4)
+    g, o = make_pair(nd, oracle, n, True)
+    g.set_custom_constraint(lambda a: np.multiply(a, 0.5, out=a))
+    f = nd.Complex2D(n, n, data=t)
+    g.apply_constraint(f)
+    to = t.copy()
+    o.apply(to)
+    assert rel(f.get(), 0.5 * to) <= 1e-12
+
+
+@pytest.mark.parametrize("alg", ["ER", "HIO"])
+def test_planar_iterations_with_constraint(nd, oracle, synth, alg):
+    n = 128
+    d = synth.planar_block(n=n, block=48, support=56)
+    est = nd.Complex2D(n, n, data=d["x0"])
+    g = nd.PlanarCDI(est)
+    xo = d["x0"].astype(np.complex128).copy()
+    o = oracle.OracleCDI(xo, oracle.PLANAR)
+    # two overlapping regions inside the support (log|x| is finite there)
+    s0 = n // 2 - 28
+    a = np.zeros((n, n))
+    a[s0 + 2:s0 + 30, s0 + 2:s0 + 54] = 1
+    b = np.zeros((n, n))
+    b[s0 + 20:s0 + 54, s0 + 2:s0 + 54] = 1
+    tg, to = make_pair(nd, oracle, n, True, regs=(a, b))
+    g.set_complex_constraint(tg)
+    o.set_complex_constraint(to)
+    for obj in (g, o):
+        obj.set_support(d["support"])
+        obj.set_intensity(d["intensity"])
+        obj.set_algorithm(getattr(oracle, alg))
+    eg = g.iterate(12)
+    eo = o.iterate_n(12)
+    assert (np.abs(eg - eo) / eo).max() <= 1e-10
+    assert rel(est.get(), xo) <= 1e-10
+    assert abs(tg.c_mean(0) - to.c_mean(0)) <= 1e-10 * abs(to.c_mean(0))
+
+
+def test_fresnel_iterations_with_constraint(nd, oracle, synth):
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
+    n = 128
+    tg = nd.TransmissionConstraint(n, n)
+    tcs = oracle.OracleConstraint(n, n)
+    inside = d["support"]
+    for tc in (tg, tcs):
+        tc.add_complex_constraint(inside, 0.5, 0.5)
+    g.set_complex_constraint(tg)
+    o.set_complex_constraint(tcs)
+    # start from a transmission function with phases well away from zero:
+    # initialise_estimate's real T makes the region constant c = sum|log|T|| /
+    # sum|arg T| divide by rounding-level phases (ill-conditioned on any two
+    # implementations)
+    rng = np.random.default_rng(5)
+    phi = -rng.uniform(0.1, 0.4, (n, n))
+    T0 = np.exp((0.3 + 1j) * phi)
+    g.set_transmission_function(T0)
+    o.set_transmission_function(T0)
+    assert rel(est.get(), xo) <= 1e-13
+    eg = g.iterate(8)
+    eo = o.iterate_n(8)
+    assert (np.abs(eg - eo) / eo).max() <= 1e-10
+    assert rel(est.get(), xo) <= 1e-10
+    # get_transmission_function applies the constraint, as in the reference
+    assert rel(g.get_transmission_function(), o.get_transmission_function()) <= 1e-10

@@ -257,8 +257,9 @@ def test_fp32_against_fp64_oracle(nd, oracle, synth):
     assert abs(e1 - o.get_error()) <= 1e-5 * o.get_error()
 
 
+@pytest.mark.parametrize("alg", ["HIO", "ER", "DM"])
 @pytest.mark.parametrize("double", [False, True], ids=["fp32", "fp64"])
-def test_tile_variants_bit_identical(nd, synth, double):
+def test_tile_variants_bit_identical(nd, synth, double, alg):
     """Every tile variant of the fused passes uses the same radix plan and
     points per thread, so the estimates are bit-identical whichever variant
     the autotuner picks (the error sums follow the tile geometry's reduction
@@ -266,13 +267,14 @@ def test_tile_variants_bit_identical(nd, synth, double):
     d = synth.planar_block(n=512, block=160, support=200)
     cdt = np.complex128 if double else np.complex64
     outs = []
-    for cv in range(3):
-        for rv in range(2):
+    ncv, nrv = nd.tile_variant_count()
+    for cv in range(ncv):
+        for rv in range(nrv):
             est = nd.Complex2D(512, 512, double=double, data=d["x0"].astype(cdt))
             g = nd.PlanarCDI(est)
             g.set_support(d["support"])
             g.set_intensity(d["intensity"])
-            g.set_algorithm(nd.HIO)
+            g.set_algorithm(getattr(nd, alg))
             g.set_tile_variants(cv, rv)
             e = g.iterate(4)
             assert g.get_tile_variants() == (cv, rv)

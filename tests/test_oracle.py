@@ -474,3 +474,123 @@ def test_wf_support_disk_and_iteration_fixed_point(oracle):
     # a field whose zone-plate-plane image equals itself times the support and
     # whose modulus is the measured one is a fixed point
     assert s.max() > 1.0  # left unnormalised like the reference
+
+
+# ------------------------------------------------ TransmissionConstraint ----
+# Pinned by what the mathematics fixes: a homogeneous material
+# t = exp((kappa + i) phi), phi < 0, is a fixed point of every complex
+# constraint and its region constant is c = kappa; alpha1 = 1 replaces the
+# magnitude by exp(c phi); charge flipping negates positive imaginary parts;
+# unity caps the magnitude at 1 keeping the phase; the first region to claim
+# a pixel keeps it.
+
+def homogeneous(n, kappa, rng):
+    phi = -rng.uniform(0.05, 1.0, (n, n))
+    return np.exp((kappa + 1j) * phi)
+
+
+def test_constraint_homogeneous_fixed_point_and_c(oracle):
+    n = 16
+    rng = np.random.default_rng(40)
+    for kappa in (0.3, 2.5):
+        t = homogeneous(n, kappa, rng)
+        tc = oracle.OracleConstraint(n, n)
+        k = tc.add_complex_constraint(np.ones((n, n)), 0.7, 0.4)
+        out = tc.apply(t.copy())
+        assert abs(tc.c_mean(k) - kappa) < 1e-13 * kappa
+        assert rel(out, t) < 1e-14
+
+
+def test_constraint_alpha1_replaces_magnitude(oracle):
+    n = 16
+    rng = np.random.default_rng(41)
+    kappa = 1.7
+    phi = rng.uniform(0.05, 1.0, (n, n))        # positive phase: not a fixed point
+    t = np.exp(-kappa * phi + 1j * phi)         # |log|t|| = kappa |phi|, so c = kappa
+    tc = oracle.OracleConstraint(n, n)
+    tc.set_charge_flipping(False)
+    tc.set_enforce_unity(False)
+    tc.add_complex_constraint(np.ones((n, n)), 1.0, 0.0)
+    out = tc.apply(t.copy())
+    assert rel(np.abs(out), np.exp(kappa * phi)) < 1e-13
+    assert np.abs(np.angle(out) - phi).max() < 1e-13
+    # alpha2 = 1 instead: phase log|t| / c = -phi, magnitude unchanged
+    tc2 = oracle.OracleConstraint(n, n)
+    tc2.set_charge_flipping(False)
+    tc2.set_enforce_unity(False)
+    tc2.add_complex_constraint(np.ones((n, n)), 0.0, 1.0)
+    out2 = tc2.apply(t.copy())
+    assert np.abs(np.angle(out2) + phi).max() < 1e-13
+    assert rel(np.abs(out2), np.abs(t)) < 1e-13
+
+
+def test_constraint_charge_flip_and_unity(oracle):
+    n = 16
+    rng = np.random.default_rng(42)
+    t = (rng.uniform(0.2, 1.6, (n, n)) * np.exp(1j * rng.uniform(-np.pi, np.pi, (n, n))))
+    tc = oracle.OracleConstraint(n, n)       # defaults: flip (sign +1) and unity
+    out = tc.apply(t.copy())
+    assert (out.imag <= 0).all()
+    small = np.abs(t) <= 1
+    assert np.array_equal(out[small], np.where(t.imag > 0, np.conj(t), t)[small])  # flip only: exact
+    big = np.abs(t) > 1
+    assert np.abs(np.abs(out[big]) - 1).max() < 1e-15
+    assert np.abs(np.abs(out[~big]) - np.abs(t[~big])).max() < 1e-15
+    flipped = np.where(t.imag > 0, np.conj(t), t)
+    assert np.abs(np.angle(out) - np.angle(flipped)).max() < 1e-14
+    tc2 = oracle.OracleConstraint(n, n)
+    tc2.set_enforce_unity(False)
+    tc2.set_charge_flipping(True, -1)         # flip negative imaginary parts instead
+    out2 = tc2.apply(t.copy())
+    assert (out2.imag >= 0).all() and rel(np.abs(out2), np.abs(t)) < 1e-15
+
+
+def test_constraint_regions_first_claim_and_fixed_c(oracle):
+    n = 16
+    rng = np.random.default_rng(43)
+    a = np.zeros((n, n)); a[:, :10] = 1
+    b = np.zeros((n, n)); b[:, 6:] = 1           # overlaps a in columns 6..9
+    ka, kb = 0.5, 3.0
+    phi = -rng.uniform(0.05, 1.0, (n, n))
+    t = np.where(a > 0, np.exp((ka + 1j) * phi), np.exp((kb + 1j) * phi))
+    tc = oracle.OracleConstraint(n, n)
+    i = tc.add_complex_constraint(a, 0.5, 0.5)
+    j = tc.add_complex_constraint(b, 0.5, 0.5)
+    out = tc.apply(t.copy())
+    # region b holds columns 10.. only, so both regions are homogeneous
+    assert abs(tc.c_mean(i) - ka) < 1e-13 and abs(tc.c_mean(j) - kb) < 1e-12
+    assert rel(out, t) < 1e-14
+    # a fixed c overrides the mean: alpha1 = 1 gives exp(c phi)
+    tc2 = oracle.OracleConstraint(n, n)
+    tc2.set_enforce_unity(False)
+    tc2.add_complex_constraint(np.ones((n, n)), 1.0, 0.0, fixed_c=2.0)
+    out2 = tc2.apply(t.copy())
+    assert tc2.c_mean(0) == 2.0
+    assert rel(np.abs(out2), np.exp(2.0 * phi)) < 1e-13
+
+
+def test_fresnel_support_applies_constraint_to_transmission(oracle):
+    """FresnelCDI::apply_support with a constraint: support, then the
+    transmission function T = (esw + Ls)/Ls is constrained and written back
+    as esw = (T - 1) Ls.  A homogeneous T inside a full support is unchanged."""
+    n = 32
+    o, x, wf, rng = fresnel(oracle, n)
+    Ls = o.get_illumination_at_sample()
+    T = homogeneous(n, 0.8, rng)
+    esw = (T - 1) * Ls
+    o.set_support(np.ones((n, n)))
+    tc = oracle.OracleConstraint(n, n)
+    tc.add_complex_constraint(np.ones((n, n)), 0.6, 0.6)
+    o.set_complex_constraint(tc)
+    e2 = esw.copy()
+    o.apply_support(e2)
+    assert rel(e2, esw) < 1e-13
+    # a non-homogeneous T moves, and get_transmission_function applies the constraint
+    T2 = T * np.exp(0.3j * rng.uniform(size=(n, n)))
+    e3 = (T2 - 1) * Ls
+    got = o.get_transmission_function(e3)
+    assert rel(got, T2) > 1e-3
+    o.apply_support(e3)
+    tc0 = oracle.OracleConstraint(n, n)  # the same constraint applied by hand
+    tc0.add_complex_constraint(np.ones((n, n)), 0.6, 0.6)
+    assert rel(e3, (tc0.apply(T2.copy()) - 1) * Ls) < 1e-12
