@@ -112,6 +112,11 @@ __device__ __forceinline__ void dft(c2_t<T> (&a)[R]) {
   }
 }
 
+// LDS position of transform position p: one pad element after every 8, so
+// the stride-8 writes of the first exchanges spread over all banks.
+template <int N> constexpr int lds_len() { return N + N / 8; }
+__device__ __forceinline__ int lpos(int p) { return p + (p >> 3); }
+
 // Twiddles of stage S for the NB = P/R butterflies of one thread:
 // w[q][m-1] = W_{L R}^{k m}, k = (t + q*TPT) mod L (sign -1 values).
 template <typename T, int N, int P, int S>
@@ -137,7 +142,7 @@ __device__ __forceinline__ void load_tw(StageTw<T, N, P, S>& w, int t, const c2_
 }
 
 // One transform of length N held by TPT = N/P threads, P points each.
-// `lds` addresses position pos of this transform at lds[pos * lstride]
+// `lds` addresses position pos of this transform at lds[lpos(pos) * lstride]
 // (the caller offsets the pointer per transform); `tw` is the plan's twiddle
 // table (sign -1 values; DIR > 0 uses their conjugates).  The twiddles of
 // stage S+1 are loaded before stage S's exchange barrier, so their latency
@@ -176,16 +181,30 @@ __device__ __forceinline__ void fft_stage(c2_t<T> (&v)[P], c2_t<T>* lds, int lst
       } else {
         const int k = j % L;
         const int base = (j - k) * R + k;
+        if constexpr (L % 8 == 0 || (L == 1 && 8 % R == 0)) {
+          // lpos(base + m*L) = lpos(base) + m*L + m*L/8: m*L is a multiple
+          // of 8, or base is a multiple of R and m < R divides 8
+          const int lb = lpos(base);
 #pragma unroll
-        for (int m = 0; m < R; ++m) lds[(base + m * L) * lstride] = a[m];
+          for (int m = 0; m < R; ++m) lds[(lb + m * L + (m * L) / 8) * lstride] = a[m];
+        } else {
+#pragma unroll
+          for (int m = 0; m < R; ++m) lds[lpos(base + m * L) * lstride] = a[m];
+        }
       }
     }
     if constexpr (!last) {
       StageTw<T, N, P, S + 1> wn;
       load_tw<T, N, P, S + 1>(wn, t, tw);
       __syncthreads();
+      if constexpr (TPT % 8 == 0) {
+        const int lt = lpos(t);
 #pragma unroll
-      for (int i = 0; i < P; ++i) v[i] = lds[(t + i * TPT) * lstride];
+        for (int i = 0; i < P; ++i) v[i] = lds[(lt + i * (TPT + TPT / 8)) * lstride];
+      } else {
+#pragma unroll
+        for (int i = 0; i < P; ++i) v[i] = lds[lpos(t + i * TPT) * lstride];
+      }
       __syncthreads();
       fft_stage<T, N, P, DIR, S + 1>(v, lds, lstride, t, tw, wn);
     }
