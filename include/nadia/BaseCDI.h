@@ -101,6 +101,14 @@ class BaseCDI {
   // 0 = fused passes (default), 1 = one kernel per reference operation
   void set_execution_path(int path);
   nadia_recon* device_handle() { return recon; }
+  // For callers that drive the device reconstruction directly
+  // (PhaseDiverseCDI): bring the estimate's device copy up to date, and
+  // record that device work changed it.
+  nadia_recon* device_sync() {
+    dev(complex);
+    return recon;
+  }
+  void device_changed() { complex.device_modified(); }
 };
 
 #endif

@@ -66,6 +66,7 @@ typedef struct nadia_field nadia_field; /* device-resident complex field */
 typedef struct nadia_recon nadia_recon; /* one reconstruction (BaseCDI object) */
 typedef struct nadia_batch nadia_batch; /* reconstructions iterated together */
 typedef struct nadia_constraint nadia_constraint; /* a TransmissionConstraint */
+typedef struct nadia_pd nadia_pd;                 /* a PhaseDiverseCDI */
 /* host function applied to a transmission function (interleaved complex,
  * x-major, nx x ny, float or double as `precision`), in place */
 typedef void (*nadia_custom_constraint_fn)(void* complex_host, int nx, int ny, int precision, void* user);
@@ -220,6 +221,31 @@ int nadia_constraint_set_custom(nadia_constraint* tc, nadia_custom_constraint_fn
 int nadia_constraint_apply(nadia_constraint* tc, nadia_field* f);
 /* BaseCDI::set_complex_constraint (NADIA/include/BaseCDI.h:471-473) */
 int nadia_recon_set_complex_constraint(nadia_recon* r, nadia_constraint* tc);
+
+/* ---- PhaseDiverseCDI (NADIA/include/PhaseDiverseCDI.h,
+ * NADIA/src/PhaseDiverseCDI.c++) ----
+ * Frames are PlanarCDI or FresnelCDI reconstructions (not owned, same
+ * precision, same device) at positions (x, y) of one global object, which
+ * grows to the frames' bounding box as they are added.  create = the
+ * constructor (beta, gamma, parallel, granularity = sub-pixel scale);
+ * add_position = add_new_position; set_parameters = set_iterations_per_cycle,
+ * set_feedback_parameter, set_amplification_factor; set_probe_scaling;
+ * initialise_estimate; iterate = one "big" iteration (frame_errors, may be
+ * NULL, receives each frame's get_error()); get/set_transmission = the
+ * global object (nx x ny, interleaved complex, x-major).  In parallel mode
+ * the frames' iterations run as one batch.  Position refinement
+ * (adjust_positions) is not provided. */
+int nadia_pd_create(double beta, double gamma, int parallel, int granularity, nadia_pd** out);
+int nadia_pd_destroy(nadia_pd* d);
+int nadia_pd_add_position(nadia_pd* d, nadia_recon* frame, double x, double y, double alpha);
+int nadia_pd_remove_positions(nadia_pd* d);
+int nadia_pd_set_parameters(nadia_pd* d, int iterations_per_cycle, double beta, double gamma);
+int nadia_pd_set_probe_scaling(nadia_pd* d, int n_probe, double alpha);
+int nadia_pd_initialise_estimate(nadia_pd* d);
+int nadia_pd_iterate(nadia_pd* d, double* frame_errors);
+int nadia_pd_get_size(const nadia_pd* d, int* nx, int* ny);
+int nadia_pd_get_transmission(nadia_pd* d, void* host);
+int nadia_pd_set_transmission(nadia_pd* d, int nx, int ny, const void* host);
 
 /* Device memory (bytes) a reconstruction holds besides its estimate field,
  * and an estimate for one not yet created, estimate field included (fused

@@ -121,6 +121,17 @@ def _sig(lib):
         "nadia_constraint_set_custom": (I, [P, P, P]),
         "nadia_constraint_apply": (I, [P, P]),
         "nadia_recon_set_complex_constraint": (I, [P, P]),
+        "nadia_pd_create": (I, [D, D, I, I, PP]),
+        "nadia_pd_destroy": (I, [P]),
+        "nadia_pd_add_position": (I, [P, P, D, D, D]),
+        "nadia_pd_remove_positions": (I, [P]),
+        "nadia_pd_set_parameters": (I, [P, I, D, D]),
+        "nadia_pd_set_probe_scaling": (I, [P, I, D]),
+        "nadia_pd_initialise_estimate": (I, [P]),
+        "nadia_pd_iterate": (I, [P, DP]),
+        "nadia_pd_get_size": (I, [P, IP, IP]),
+        "nadia_pd_get_transmission": (I, [P, P]),
+        "nadia_pd_set_transmission": (I, [P, I, I, P]),
         "nadia_recon_bytes_estimate": (I, [I, I, I, I, I, ctypes.POINTER(ctypes.c_size_t)]),
         "nadia_recon_set_tile_variants": (I, [P, I, I]),
         "nadia_recon_get_tile_variants": (I, [P, IP, IP]),
@@ -457,6 +468,74 @@ class FresnelCDI_WF(BaseCDI):
 
     def set_support(self, z_diameter, size=1.01):  # hides BaseCDI::set_support like the reference
         _check(library().nadia_wf_set_support(self.h, float(z_diameter), float(size)))
+
+
+class PhaseDiverseCDI:
+    """PhaseDiverseCDI (NADIA/include/PhaseDiverseCDI.h) over nadia_pd: frames
+    (PlanarCDI / FresnelCDI objects) at positions of one global object."""
+
+    def __init__(self, beta=1.0, gamma=1.0, parallel=False, granularity=1):
+        h = ctypes.c_void_p()
+        _check(library().nadia_pd_create(float(beta), float(gamma), int(bool(parallel)), int(granularity),
+                                         ctypes.byref(h)))
+        self.h = h
+        self.beta, self.gamma, self.ipc = float(beta), float(gamma), 1
+        self.frames = []
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.nadia_pd_destroy(self.h)
+            self.h = None
+
+    def add_new_position(self, local, x=0.0, y=0.0, alpha=1.0):
+        _check(library().nadia_pd_add_position(self.h, local.h, float(x), float(y), float(alpha)))
+        self.frames.append(local)
+
+    def remove_positions(self):
+        _check(library().nadia_pd_remove_positions(self.h))
+        self.frames = []
+
+    def _params(self):
+        _check(library().nadia_pd_set_parameters(self.h, int(self.ipc), self.beta, self.gamma))
+
+    def set_iterations_per_cycle(self, n):
+        self.ipc = int(n)
+        self._params()
+
+    def set_feedback_parameter(self, beta):
+        self.beta = float(beta)
+        self._params()
+
+    def set_amplification_factor(self, gamma):
+        self.gamma = float(gamma)
+        self._params()
+
+    def set_probe_scaling(self, n_probe, alpha):
+        _check(library().nadia_pd_set_probe_scaling(self.h, int(n_probe), float(alpha)))
+
+    def initialise_estimate(self):
+        _check(library().nadia_pd_initialise_estimate(self.h))
+
+    def iterate(self):
+        """One big iteration; returns each frame's get_error()."""
+        e = np.zeros(len(self.frames))
+        _check(library().nadia_pd_iterate(self.h, e.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return e
+
+    def size(self):
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        _check(library().nadia_pd_get_size(self.h, ctypes.byref(nx), ctypes.byref(ny)))
+        return nx.value, ny.value
+
+    def get_transmission(self):
+        nx, ny = self.size()
+        out = np.empty((nx, ny), self.frames[0].cdt)
+        _check(library().nadia_pd_get_transmission(self.h, _ptr(out)))
+        return out
+
+    def set_transmission(self, t):
+        t = np.ascontiguousarray(t, dtype=self.frames[0].cdt)
+        _check(library().nadia_pd_set_transmission(self.h, t.shape[0], t.shape[1], _ptr(t)))
 
 
 CUSTOM_CONSTRAINT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,

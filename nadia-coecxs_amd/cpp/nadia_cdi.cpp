@@ -17,6 +17,7 @@
 #include "PlanarCDI.h"
 #include "TransmissionConstraint.h"
 #include "io.h"
+#include "PhaseDiverseCDI.h"
 
 using std::cout;
 using std::endl;
@@ -556,4 +557,104 @@ int write_cplx(std::string file_name, const Complex_2D& complex) {
   fwrite(buffer.data(), sizeof(double), buffer.size(), file);
   fclose(file);
   return SUCCESS;
+}
+
+// ---- PhaseDiverseCDI (NADIA/src/PhaseDiverseCDI.c++) ------------------------
+
+PhaseDiverseCDI::PhaseDiverseCDI(double beta, double gamma, bool parallel, int granularity)
+    : beta(beta), gamma(gamma), object(0), iterations_per_cycle(1), scale(granularity), parallel(parallel), h(0) {
+  nadia_cxx_check(nadia_pd_create(beta, gamma, parallel ? 1 : 0, granularity, &h), "PhaseDiverseCDI");
+}
+
+PhaseDiverseCDI::~PhaseDiverseCDI() {
+  if (object) delete object;
+  nadia_pd_destroy(h);
+}
+
+void PhaseDiverseCDI::push_parameters() {
+  nadia_cxx_check(nadia_pd_set_parameters(h, iterations_per_cycle, beta, gamma), "PhaseDiverseCDI parameters");
+}
+
+// the frames' estimates were changed on the device
+void PhaseDiverseCDI::frames_changed() {
+  for (size_t i = 0; i < singleCDI.size(); i++) singleCDI[i]->device_changed();
+}
+
+void PhaseDiverseCDI::add_new_position(BaseCDI* local, double x, double y, double a) {
+  nadia_cxx_check(nadia_pd_add_position(h, local->device_sync(), x, y, a), "add_new_position");
+  singleCDI.push_back(local);
+  x_position.push_back(x);
+  y_position.push_back(y);
+  alpha.push_back(a);
+  cout << "Added position " << singleCDI.size() - 1 << endl;
+}
+
+void PhaseDiverseCDI::remove_positions() {
+  nadia_cxx_check(nadia_pd_remove_positions(h), "remove_positions");
+  singleCDI.clear();
+  x_position.clear();
+  y_position.clear();
+  alpha.clear();
+}
+
+void PhaseDiverseCDI::initialise_estimate() {
+  for (size_t i = 0; i < singleCDI.size(); i++) singleCDI[i]->device_sync();
+  nadia_cxx_check(nadia_pd_initialise_estimate(h), "initialise_estimate");
+}
+
+void PhaseDiverseCDI::iterate() {
+  static int total_iterations = 0;
+  cout << "Iteration " << total_iterations << endl;
+  for (size_t i = 0; i < singleCDI.size(); i++) singleCDI[i]->device_sync();
+  std::vector<double> errors(singleCDI.size());
+  nadia_cxx_check(nadia_pd_iterate(h, errors.data()), "PhaseDiverseCDI::iterate");
+  frames_changed();
+  for (size_t i = 0; i < errors.size(); i++) cout << "Error for frame " << i << " is " << errors[i] << endl;
+  total_iterations++;
+}
+
+void PhaseDiverseCDI::set_iterations_per_cycle(int iterations) {
+  iterations_per_cycle = iterations;
+  push_parameters();
+}
+
+void PhaseDiverseCDI::set_feedback_parameter(double b) {
+  beta = b;
+  push_parameters();
+}
+
+void PhaseDiverseCDI::set_amplification_factor(double g) {
+  gamma = g;
+  push_parameters();
+}
+
+void PhaseDiverseCDI::set_probe_scaling(int n_probe, double a) {
+  nadia_cxx_check(nadia_pd_set_probe_scaling(h, n_probe, a), "set_probe_scaling");
+  if (n_probe >= 0 && n_probe < (int)alpha.size()) alpha[n_probe] = a;
+}
+
+Complex_2D* PhaseDiverseCDI::get_transmission() {
+  int nx = 0, ny = 0;
+  nadia_cxx_check(nadia_pd_get_size(h, &nx, &ny), "get_transmission");
+  if (nx == 0 || ny == 0) return 0;
+  if (!object || object->get_size_x() != nx || object->get_size_y() != ny) {
+    if (object) delete object;
+    object = new Complex_2D(nx, ny);
+  }
+  nadia_cxx_check(nadia_pd_get_transmission(h, object->host_data_for_write()), "get_transmission");
+  return object;
+}
+
+void PhaseDiverseCDI::set_transmission(Complex_2D& t) {
+  int rc = nadia_pd_set_transmission(h, t.get_size_x(), t.get_size_y(), t.host_data());
+  if (rc == NADIA_ESIZE)
+    cout << "Can not set the transmission function in PhaseDiverseCDI because the complex array given does not "
+         << "have the same dimensions as the current transmission function" << endl;
+  else
+    nadia_cxx_check(rc, "set_transmission");
+}
+
+void PhaseDiverseCDI::adjust_positions(int, bool, int, int, int, int, double) {
+  cout << "PhaseDiverseCDI::adjust_positions: position refinement is not available in this build; "
+       << "the positions are unchanged" << endl;
 }

@@ -213,6 +213,26 @@ struct nadia_constraint {
   void* user;
 };
 
+struct PdFrame {
+  nadia_recon* r;
+  double x, y, alpha;
+  void* result;  // the frame's transmission (Fresnel) or estimate (planar), lnx x lny
+  void* weight;  // lnx x lny, element type of the frame
+};
+
+struct nadia_pd {
+  int dev, prec;
+  double beta, gamma;
+  int parallel, scale, ipc;
+  std::vector<PdFrame> frames;
+  void* object;
+  int nx, ny, x_min, y_min;
+  bool weights_set;
+  PdFrameDev* dframes;  // device copy of the frame geometry (scale_object)
+  int dframes_cap;
+  nadia_batch* batch;   // parallel mode: the frames' iterations run as one batch
+};
+
 struct nadia_batch {
   int dev;
   std::vector<nadia_recon*> jobs;
@@ -1886,6 +1906,406 @@ int nadia_recon_set_complex_constraint(nadia_recon* r, nadia_constraint* tc) {
   if (tc && tc->dev != r->dev) return fail(NADIA_EINVAL, "constraint lives on another device");
   r->tc = tc;
   r->w_valid = r->w2_valid = false;
+  return 0;
+}
+
+// ---- PhaseDiverseCDI (NADIA/src/PhaseDiverseCDI.c++) ---------------------
+
+}  // extern "C"
+
+namespace {
+
+int pd_gx(const nadia_pd* d, int x, double off) { return (int)(x - off - d->x_min); }
+int pd_gy(const nadia_pd* d, int y, double off) { return (int)(y - off - d->y_min); }
+
+// reallocate_object_memory (PhaseDiverseCDI.c++:72-82): a fresh zero array
+int pd_realloc(nadia_pd* d, int nx, int ny) {
+  DeviceCtx* c = dev_ctx(d->dev);
+  if (d->object) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(d->object);
+    d->object = nullptr;
+  }
+  const size_t b = (size_t)nx * ny * (d->prec == NADIA_FP64 ? 16 : 8);
+  int rc;
+  if ((rc = alloc(&d->object, b))) return rc;
+  HIPCHK(hipMemsetAsync(d->object, 0, b, c->stream));
+  d->nx = nx;
+  d->ny = ny;
+  return 0;
+}
+
+template <typename T>
+std::vector<double> download_real(nadia_recon* r, const void* dev) {
+  std::vector<T> h(r->npix);
+  (void)hipMemcpyAsync(h.data(), dev, r->npix * sizeof(T), hipMemcpyDeviceToHost, S_(r));
+  (void)hipStreamSynchronize(S_(r));
+  return std::vector<double>(h.begin(), h.end());
+}
+
+// |illumination at sample| in the element type, as Complex_2D::get_2d(MAG)
+template <typename T>
+std::vector<double> illum_mag(nadia_recon* r) {
+  std::vector<T> h(2 * r->npix);
+  (void)hipMemcpyAsync(h.data(), r->Ls, r->npix * 2 * sizeof(T), hipMemcpyDeviceToHost, S_(r));
+  (void)hipStreamSynchronize(S_(r));
+  std::vector<double> m(r->npix);
+  for (long long p = 0; p < r->npix; p++) {
+    const T a = h[2 * p], b = h[2 * p + 1];
+    m[p] = (double)(T)std::sqrt(a * a + b * b);
+  }
+  return m;
+}
+
+// set_up_weights (PhaseDiverseCDI.c++:721-848), on the host in double with
+// the results stored in the frames' element type
+template <typename T>
+int pd_weights_T(nadia_pd* d) {
+  const int nf = (int)d->frames.size();
+  std::vector<std::vector<double>> w(nf);
+  for (int n = 0; n < nf; n++) {
+    nadia_recon* r = d->frames[n].r;
+    const std::vector<double> sup = download_real<T>(r, r->S);
+    std::vector<double> mag;
+    double mx = 0;
+    const bool fres = r->kind == NADIA_FRESNEL;
+    if (fres) {
+      mag = illum_mag<T>(r);
+      mx = mag[0];
+      for (double v : mag) mx = v > mx ? v : mx;
+    }
+    w[n].resize(r->npix);
+    for (long long p = 0; p < r->npix; p++) {
+      double value = fres ? mag[p] / mx : 1;
+      value = d->frames[n].alpha * std::pow(value, d->gamma);
+      if (sup[p] <= 0.0) value = 0;
+      if (!d->parallel) value *= d->beta;
+      w[n][p] = (double)(T)value;
+    }
+  }
+  if (d->parallel) {
+    std::vector<double> norm((size_t)d->nx * d->ny, 0.0);
+    for (int n = 0; n < nf; n++) {
+      const nadia_recon* r = d->frames[n].r;
+      for (int i_ = 0; i_ < r->nx; i_++)
+        for (int j_ = 0; j_ < r->ny; j_++) {
+          const int i = pd_gx(d, i_, d->frames[n].x), j = pd_gy(d, j_, d->frames[n].y);
+          if (i >= 0 && j >= 0 && i < d->nx && j < d->ny) {
+            double& g = norm[(size_t)i * d->ny + j];
+            const double v = w[n][(size_t)i_ * r->ny + j_];
+            g = (double)(T)(n == 0 ? v : v + g);
+          }
+        }
+    }
+    for (int n = 0; n < nf; n++) {
+      const nadia_recon* r = d->frames[n].r;
+      for (int i_ = 0; i_ < r->nx; i_++)
+        for (int j_ = 0; j_ < r->ny; j_++) {
+          const int i = pd_gx(d, i_, d->frames[n].x), j = pd_gy(d, j_, d->frames[n].y);
+          if (i >= 0 && j >= 0 && i < d->nx && j < d->ny) {
+            double& v = w[n][(size_t)i_ * r->ny + j_];
+            const double nm = norm[(size_t)i * d->ny + j];
+            v = nm <= 0 ? 0 : (double)(T)(d->beta * v / nm);
+          }
+        }
+    }
+  }
+  DeviceCtx* c = dev_ctx(d->dev);
+  for (int n = 0; n < nf; n++) {
+    std::vector<T> h(w[n].begin(), w[n].end());
+    HIPCHK(hipMemcpyAsync(d->frames[n].weight, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+int pd_weights(nadia_pd* d) {
+  if (d->weights_set) return 0;
+  d->weights_set = true;
+  int rc;
+  DISPATCH(d->prec, rc = pd_weights_T<T>(d));
+  if (rc) return rc;
+  // frame geometry for scale_object
+  const int nf = (int)d->frames.size();
+  DeviceCtx* c = dev_ctx(d->dev);
+  if (d->dframes_cap < nf) {
+    if (d->dframes) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      (void)hipFree(d->dframes);
+      d->dframes = nullptr;
+    }
+    if ((rc = alloc((void**)&d->dframes, sizeof(PdFrameDev) * (nf > 0 ? nf : 1)))) return rc;
+    d->dframes_cap = nf;
+  }
+  std::vector<PdFrameDev> h(nf);
+  for (int n = 0; n < nf; n++) h[n] = PdFrameDev{d->frames[n].weight, d->frames[n].r->nx, d->frames[n].r->ny, d->frames[n].x, d->frames[n].y};
+  if (nf) {
+    HIPCHK(hipMemcpyAsync(d->dframes, h.data(), sizeof(PdFrameDev) * nf, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+PdArgs pd_args(nadia_pd* d, int n) {
+  PdArgs a{};
+  a.object = d->object;
+  a.nx = d->nx;
+  a.ny = d->ny;
+  a.scale = d->scale;
+  a.x_min = d->x_min;
+  a.y_min = d->y_min;
+  a.parallel = d->parallel;
+  if (n >= 0) {
+    const PdFrame& f = d->frames[n];
+    a.f = PdFrameDev{f.weight, f.r->nx, f.r->ny, f.x, f.y};
+    a.small = f.result;
+  }
+  return a;
+}
+
+// get_result / set_result (PhaseDiverseCDI.c++:317-335)
+int pd_get_result(nadia_pd* d, int n) {
+  nadia_recon* r = d->frames[n].r;
+  void* res = d->frames[n].result;
+  if (r->kind == NADIA_FRESNEL) {
+    DISPATCH(r->prec, k_trans_from_esw<T>(r->est->d, r->Ls, res, r->npix, S_(r)));
+    LAUNCHCHK();
+    if (r->tc) return tc_apply(r->tc, S_(r), res, r->prec);
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(res, r->est->d, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+  return 0;
+}
+
+int pd_set_result(nadia_pd* d, int n) {
+  nadia_recon* r = d->frames[n].r;
+  const void* res = d->frames[n].result;
+  if (r->kind == NADIA_FRESNEL) {
+    DISPATCH(r->prec, k_esw_from_trans<T>(res, r->Ls, r->est->d, r->nx, r->ny, S_(r)));
+    LAUNCHCHK();
+  } else {
+    HIPCHK(hipMemcpyAsync(r->est->d, res, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+  }
+  r->est->version++;
+  return 0;
+}
+
+// add_to_object (PhaseDiverseCDI.c++:851-1016)
+int pd_add(nadia_pd* d, int n) {
+  int rc;
+  if ((rc = pd_get_result(d, n)) || (rc = pd_weights(d))) return rc;
+  const PdArgs a = pd_args(d, n);
+  DISPATCH(d->prec, k_pd_add<T>(a, dev_ctx(d->dev)->stream));
+  LAUNCHCHK();
+  return 0;
+}
+
+// update_from_object (PhaseDiverseCDI.c++:1066-1130)
+int pd_update(nadia_pd* d, int n) {
+  int rc;
+  if ((rc = pd_weights(d))) return rc;
+  const PdArgs a = pd_args(d, n);
+  DISPATCH(d->prec, k_pd_update<T>(a, dev_ctx(d->dev)->stream));
+  LAUNCHCHK();
+  return pd_set_result(d, n);
+}
+
+// scale_object (PhaseDiverseCDI.c++:270-313)
+int pd_scale(nadia_pd* d, double factor) {
+  int rc;
+  if ((rc = pd_weights(d))) return rc;
+  const PdArgs a = pd_args(d, -1);
+  DISPATCH(d->prec, k_pd_scale<T>(a, d->dframes, (int)d->frames.size(), factor, dev_ctx(d->dev)->stream));
+  LAUNCHCHK();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nadia_pd_create(double beta, double gamma, int parallel, int granularity, nadia_pd** out) {
+  if (!out || granularity < 1) return fail(NADIA_EINVAL, "bad argument");
+  nadia_pd* d = new nadia_pd{};
+  d->dev = current_device();
+  d->prec = -1;
+  d->beta = beta;
+  d->gamma = gamma;
+  d->parallel = parallel ? 1 : 0;
+  d->scale = granularity;
+  d->ipc = 1;
+  *out = d;
+  return 0;
+}
+
+int nadia_pd_destroy(nadia_pd* d) {
+  if (!d) return 0;
+  DeviceCtx* c = dev_ctx(d->dev);
+  if (c) (void)hipStreamSynchronize(c->stream);
+  for (PdFrame& f : d->frames) {
+    if (f.result) (void)hipFree(f.result);
+    if (f.weight) (void)hipFree(f.weight);
+  }
+  if (d->object) (void)hipFree(d->object);
+  if (d->dframes) (void)hipFree(d->dframes);
+  if (d->batch) nadia_batch_destroy(d->batch);
+  delete d;
+  return 0;
+}
+
+// add_new_position (PhaseDiverseCDI.c++:118-178)
+int nadia_pd_add_position(nadia_pd* d, nadia_recon* r, double x, double y, double alpha) {
+  if (!d || !r) return fail(NADIA_EINVAL, "null argument");
+  if (r->kind == NADIA_FRESNEL_WF) return fail(NADIA_EINVAL, "frames are PlanarCDI or FresnelCDI objects");
+  if (r->dev != d->dev) return fail(NADIA_EINVAL, "frame lives on another device");
+  if (d->prec >= 0 && r->prec != d->prec) return fail(NADIA_EINVAL, "all frames must have one precision");
+  if (r->kind == NADIA_FRESNEL && !r->Ls) return fail(NADIA_ESTATE, "FresnelCDI frame not set up");
+  d->prec = r->prec;
+  d->weights_set = false;
+  const int lnx = r->nx, lny = r->ny;
+  PdFrame f{r, x, y, alpha, nullptr, nullptr};
+  int rc;
+  if ((rc = alloc(&f.result, r->npix * r->esz)) || (rc = alloc(&f.weight, r->npix * r->rsz))) {
+    if (f.result) (void)hipFree(f.result);
+    return rc;
+  }
+  DeviceCtx* c = dev_ctx(d->dev);
+  HIPCHK(hipMemsetAsync(f.result, 0, r->npix * r->esz, c->stream));
+  d->frames.push_back(f);
+  if (!d->object) {
+    d->x_min = (int)(-x);
+    d->y_min = (int)(-y);
+    if ((rc = pd_realloc(d, lnx * d->scale, lny * d->scale))) return rc;
+  }
+  int extra_x = 0, extra_y = 0;
+  const int gxmin = pd_gx(d, 0, x) * d->scale, gymin = pd_gy(d, 0, y) * d->scale;
+  const int gxmax = pd_gx(d, lnx, x) * d->scale, gymax = pd_gy(d, lny, y) * d->scale;
+  if (gxmin < 0) {
+    d->x_min = (int)(-x);
+    extra_x += -gxmin;
+  }
+  if (gymin < 0) {
+    d->y_min = (int)(-y);
+    extra_y += -gymin;
+  }
+  if (gxmax > d->nx) extra_x += gxmax - d->nx;
+  if (gymax > d->ny) extra_y += gymax - d->ny;
+  if (extra_x || extra_y) return pd_realloc(d, d->nx + extra_x, d->ny + extra_y);
+  return 0;
+}
+
+int nadia_pd_remove_positions(nadia_pd* d) {
+  if (!d) return fail(NADIA_EINVAL, "null argument");
+  DeviceCtx* c = dev_ctx(d->dev);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (PdFrame& f : d->frames) {
+    (void)hipFree(f.result);
+    (void)hipFree(f.weight);
+  }
+  d->frames.clear();
+  d->weights_set = false;
+  if (d->batch) {
+    nadia_batch_destroy(d->batch);
+    d->batch = nullptr;
+  }
+  return 0;
+}
+
+int nadia_pd_set_parameters(nadia_pd* d, int iterations_per_cycle, double beta, double gamma) {
+  if (!d || iterations_per_cycle < 0) return fail(NADIA_EINVAL, "bad argument");
+  d->ipc = iterations_per_cycle;
+  if (beta != d->beta || gamma != d->gamma) d->weights_set = false;
+  d->beta = beta;
+  d->gamma = gamma;
+  return 0;
+}
+
+int nadia_pd_set_probe_scaling(nadia_pd* d, int n, double alpha) {
+  if (!d) return fail(NADIA_EINVAL, "null argument");
+  if (n < 0 || n >= (int)d->frames.size()) {
+    fprintf(stdout, "In PhaseDiverseCDI::set_probe_scaling, the probe number given is too large. Are you trying "
+                    "to set the value of alpha before calling add_new_position?\n");
+    return 0;
+  }
+  d->frames[n].alpha = alpha;
+  d->weights_set = false;
+  return 0;
+}
+
+// initialise_estimate (PhaseDiverseCDI.c++:182-199)
+int nadia_pd_initialise_estimate(nadia_pd* d) {
+  if (!d || !d->object) return fail(NADIA_ESTATE, "no frames");
+  const PdArgs a = pd_args(d, -1);
+  // object = 1 + 0i: scale_object with no frame covering any pixel
+  DISPATCH(d->prec, k_pd_scale<T>(a, nullptr, 0, 0.0, dev_ctx(d->dev)->stream));
+  LAUNCHCHK();
+  int rc;
+  for (int n = 0; n < (int)d->frames.size(); n++)
+    if ((rc = pd_add(d, n))) return rc;
+  return 0;
+}
+
+// iterate (PhaseDiverseCDI.c++:202-267).  In parallel mode the frames'
+// iterations are independent of each other and run as one batch.
+int nadia_pd_iterate(nadia_pd* d, double* frame_errors) {
+  if (!d || !d->object) return fail(NADIA_ESTATE, "no frames");
+  const int nf = (int)d->frames.size();
+  int rc;
+  if (d->parallel) {
+    for (int n = 0; n < nf; n++)
+      if ((rc = pd_update(d, n))) return rc;
+    if (d->ipc > 0) {
+      if (!d->batch) {
+        if ((rc = nadia_batch_create(&d->batch))) return rc;
+        for (PdFrame& f : d->frames)
+          if ((rc = nadia_batch_add(d->batch, f.r))) return rc;
+      }
+      if ((rc = nadia_batch_iterate(d->batch, d->ipc, nullptr))) return rc;
+    }
+    if ((rc = pd_scale(d, 1 - d->beta))) return rc;
+    for (int n = 0; n < nf; n++)
+      if ((rc = pd_add(d, n))) return rc;
+  } else {
+    for (int n = 0; n < nf; n++) {
+      if ((rc = pd_update(d, n))) return rc;
+      if (d->ipc > 0 && (rc = nadia_recon_iterate(d->frames[n].r, d->ipc, nullptr))) return rc;
+      if ((rc = pd_add(d, n))) return rc;
+    }
+  }
+  if (frame_errors)
+    for (int n = 0; n < nf; n++)
+      if ((rc = fetch_error(d->frames[n].r, &frame_errors[n]))) return rc;
+  return 0;
+}
+
+int nadia_pd_get_size(const nadia_pd* d, int* nx, int* ny) {
+  if (!d || !nx || !ny) return fail(NADIA_EINVAL, "null argument");
+  *nx = d->nx;
+  *ny = d->ny;
+  return 0;
+}
+
+int nadia_pd_get_transmission(nadia_pd* d, void* host) {
+  if (!d || !host || !d->object) return fail(NADIA_EINVAL, "no object");
+  DeviceCtx* c = dev_ctx(d->dev);
+  HIPCHK(hipMemcpyAsync(host, d->object, (size_t)d->nx * d->ny * (d->prec == NADIA_FP64 ? 16 : 8),
+                        hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// set_transmission (PhaseDiverseCDI.c++:94-113)
+int nadia_pd_set_transmission(nadia_pd* d, int nx, int ny, const void* host) {
+  if (!d || !host) return fail(NADIA_EINVAL, "null argument");
+  if (d->prec < 0) return fail(NADIA_ESTATE, "add a frame first (it sets the precision)");
+  int rc;
+  if (!d->object && (rc = pd_realloc(d, nx, ny))) return rc;
+  if (nx != d->nx || ny != d->ny)
+    return fail(NADIA_ESIZE, "the transmission function does not have the dimensions of the object");
+  DeviceCtx* c = dev_ctx(d->dev);
+  HIPCHK(hipMemcpyAsync(d->object, host, (size_t)nx * ny * (d->prec == NADIA_FP64 ? 16 : 8), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
 

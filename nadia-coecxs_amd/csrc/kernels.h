@@ -99,6 +99,23 @@ struct TcDev {
   int charge_flip, flip_sign, unity;
 };
 
+// PhaseDiverseCDI frame geometry (NADIA/src/PhaseDiverseCDI.c++): local
+// pixel (i_, j_) of a frame at (x, y) is global pixel
+// (trunc(i_ - x - x_min) * scale, trunc(j_ - y - y_min) * scale)
+struct PdFrameDev {
+  const void* weight;  // lnx x lny, element type T
+  int lnx, lny;
+  double x, y;
+};
+
+struct PdArgs {
+  void* object;        // nx x ny complex
+  int nx, ny, scale, x_min, y_min;
+  int parallel;
+  PdFrameDev f;        // the frame (add / update)
+  void* small;         // the frame's result (lnx x lny complex)
+};
+
 // element-wise and reduction kernels (kern_misc.inc)
 template <typename T> void k_invert(void* buf, int nx, int ny, int scale, hipStream_t st);
 template <typename T> void k_scale_intensity(void* buf, const void* q, int nx, int ny, double* partials, int nblocks, hipStream_t st);
@@ -128,6 +145,11 @@ template <typename T> void k_wf_disk(void* s, int nx, int ny, double centre, dou
 template <typename T> void k_tc_stats(const void* t, const TcDev& d, long long n, double* partials, hipStream_t st);
 void k_tc_final(const TcDev& d, const double* partials, int nblocks, hipStream_t st);
 template <typename T> void k_tc_apply(void* t, const TcDev& d, long long n, hipStream_t st);
+// PhaseDiverseCDI add_to_object / update_from_object / scale_object
+template <typename T> void k_pd_add(const PdArgs& a, hipStream_t st);
+template <typename T> void k_pd_update(const PdArgs& a, hipStream_t st);
+template <typename T> void k_pd_scale(const PdArgs& a, const PdFrameDev* frames, int nframes, double factor,
+                                      hipStream_t st);
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st);
 // per-job final error reduction of the white-field row pass (grid = njobs)
 void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st);

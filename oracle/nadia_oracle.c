@@ -927,3 +927,267 @@ void ocdi_iterate_n(ocdi *o, int n, double *errors) {
 }
 
 int oracle_real_size(void) { return (int)sizeof(real); }
+
+/* ------------------------------------------------------------------------ */
+/* PhaseDiverseCDI: NADIA/src/PhaseDiverseCDI.c++ with                       */
+/* NADIA/include/PhaseDiverseCDI.h (position refinement not modelled)        */
+/* ------------------------------------------------------------------------ */
+
+#define OPD_MAX 64
+typedef struct {
+  double beta, gamma;
+  int parallel, scale, ipc;
+  int nframes;
+  ocdi *frame[OPD_MAX];
+  double xpos[OPD_MAX], ypos[OPD_MAX], alpha[OPD_MAX];
+  cplx *result[OPD_MAX];
+  real *weight[OPD_MAX];
+  cplx *object;
+  int nx, ny, x_min, y_min, weights_set;
+} opd;
+
+/* constructor: PhaseDiverseCDI.c++:27-42 */
+opd *opd_create(double beta, double gamma, int parallel, int granularity) {
+  opd *d = (opd *)calloc(1, sizeof(opd));
+  d->beta = beta; d->gamma = gamma; d->parallel = parallel; d->scale = granularity; d->ipc = 1;
+  return d;
+}
+
+void opd_destroy(opd *d) {
+  if (!d) return;
+  for (int n = 0; n < d->nframes; n++) { free(d->result[n]); free(d->weight[n]); }
+  free(d->object);
+  free(d);
+}
+
+void opd_set_iterations_per_cycle(opd *d, int n) { d->ipc = n; }
+void opd_set_feedback_parameter(opd *d, double beta) { d->beta = beta; d->weights_set = 0; }
+void opd_set_amplification_factor(opd *d, double gamma) { d->gamma = gamma; d->weights_set = 0; }
+void opd_set_probe_scaling(opd *d, int n, double alpha) {
+  if (n < d->nframes) d->alpha[n] = alpha;
+  d->weights_set = 0;
+}
+
+/* get_global_x_pos / get_local_x_pos (PhaseDiverseCDI.h): int results of
+ * double expressions (truncation toward zero) */
+static int pd_gx(opd *d, int x, double off) { return (int)(x - off - d->x_min); }
+static int pd_gy(opd *d, int y, double off) { return (int)(y - off - d->y_min); }
+static int pd_lx(opd *d, int x, double off) { return (int)(x + off + d->x_min); }
+static int pd_ly(opd *d, int y, double off) { return (int)(y + off + d->y_min); }
+
+/* reallocate_object_memory: a fresh zero array (PhaseDiverseCDI.c++:72-82) */
+static void pd_realloc(opd *d, int nx, int ny) {
+  free(d->object);
+  d->object = calloc_c((size_t)nx * ny);
+  d->nx = nx; d->ny = ny;
+}
+
+/* add_new_position: PhaseDiverseCDI.c++:118-178 */
+int opd_add_position(opd *d, ocdi *local, double x, double y, double alpha) {
+  if (d->nframes >= OPD_MAX) return -1;
+  d->weights_set = 0;
+  int lnx = local->nx, lny = local->ny, n = d->nframes++;
+  d->frame[n] = local; d->xpos[n] = x; d->ypos[n] = y; d->alpha[n] = alpha;
+  d->result[n] = calloc_c((size_t)lnx * lny);
+  d->weight[n] = calloc_r((size_t)lnx * lny);
+  if (!d->object) {
+    d->x_min = (int)(-x); d->y_min = (int)(-y);
+    pd_realloc(d, lnx * d->scale, lny * d->scale);
+  }
+  int extra_x = 0, extra_y = 0;
+  int gxmin = pd_gx(d, 0, x) * d->scale, gymin = pd_gy(d, 0, y) * d->scale;
+  int gxmax = pd_gx(d, lnx, x) * d->scale, gymax = pd_gy(d, lny, y) * d->scale;
+  if (gxmin < 0) { d->x_min = (int)(-x); extra_x += -gxmin; }
+  if (gymin < 0) { d->y_min = (int)(-y); extra_y += -gymin; }
+  if (gxmax > d->nx) extra_x += gxmax - d->nx;
+  if (gymax > d->ny) extra_y += gymax - d->ny;
+  if (extra_x || extra_y) pd_realloc(d, d->nx + extra_x, d->ny + extra_y);
+  return n;
+}
+
+/* set_up_weights: PhaseDiverseCDI.c++:721-848 */
+static void pd_weights(opd *d) {
+  if (d->weights_set) return;
+  d->weights_set = 1;
+  for (int n = 0; n < d->nframes; n++) {
+    ocdi *f = d->frame[n];
+    int lnx = f->nx, lny = f->ny;
+    real mx = 0;
+    int fres = f->kind == KIND_FRESNEL;
+    if (fres) {
+      mx = c_mag(f->illum_at_sample[0]);
+      for (size_t p = 0; p < (size_t)lnx * lny; p++) { real m = c_mag(f->illum_at_sample[p]); if (m > mx) mx = m; }
+    }
+    for (int i = 0; i < lnx; i++)
+      for (int j = 0; j < lny; j++) {
+        size_t p = (size_t)i * lny + j;
+        double value = fres ? (double)c_mag(f->illum_at_sample[p]) / mx : 1;
+        value = d->alpha[n] * pow(value, d->gamma);
+        if (f->support[p] <= 0.0) value = 0;
+        if (!d->parallel) value *= d->beta;
+        d->weight[n][p] = (real)value;
+      }
+  }
+  if (d->parallel) {
+    real *norm = calloc_r((size_t)d->nx * d->ny);
+    for (int n = 0; n < d->nframes; n++) {
+      int lnx = d->frame[n]->nx, lny = d->frame[n]->ny;
+      for (int i_ = 0; i_ < lnx; i_++)
+        for (int j_ = 0; j_ < lny; j_++) {
+          int i = pd_gx(d, i_, d->xpos[n]), j = pd_gy(d, j_, d->ypos[n]);
+          if (i >= 0 && j >= 0 && i < d->nx && j < d->ny) {
+            size_t g = (size_t)i * d->ny + j;
+            double w = d->weight[n][(size_t)i_ * lny + j_];
+            norm[g] = (real)(n == 0 ? w : w + norm[g]);
+          }
+        }
+    }
+    for (int n = 0; n < d->nframes; n++) {
+      int lnx = d->frame[n]->nx, lny = d->frame[n]->ny;
+      for (int i_ = 0; i_ < lnx; i_++)
+        for (int j_ = 0; j_ < lny; j_++) {
+          int i = pd_gx(d, i_, d->xpos[n]), j = pd_gy(d, j_, d->ypos[n]);
+          if (i >= 0 && j >= 0 && i < d->nx && j < d->ny) {
+            size_t p = (size_t)i_ * lny + j_;
+            double nm = norm[(size_t)i * d->ny + j];
+            d->weight[n][p] = (real)(nm <= 0 ? 0 : d->beta * d->weight[n][p] / nm);
+          }
+        }
+    }
+    free(norm);
+  }
+}
+
+/* get_result / set_result: PhaseDiverseCDI.c++:317-335 */
+static void pd_get_result(opd *d, int n) {
+  ocdi *f = d->frame[n];
+  if (f->kind == KIND_FRESNEL) ocdi_get_transmission_function(f, d->result[n], NULL);
+  else memcpy(d->result[n], f->complex, sizeof(cplx) * NPIX(f));
+}
+static void pd_set_result(opd *d, int n) {
+  ocdi *f = d->frame[n];
+  if (f->kind == KIND_FRESNEL) ocdi_set_transmission_function(f, d->result[n], NULL);
+  else memcpy(f->complex, d->result[n], sizeof(cplx) * NPIX(f));
+}
+
+/* add_to_object: PhaseDiverseCDI.c++:851-1016.  Sub-pixel points outside the
+ * object are skipped (the reference does not check them). */
+static void pd_add(opd *d, int n) {
+  pd_get_result(d, n);
+  pd_weights(d);
+  double xo = d->xpos[n], yo = d->ypos[n];
+  cplx *sm = d->result[n];
+  int lnx = d->frame[n]->nx, lny = d->frame[n]->ny, sc = d->scale;
+  for (int i_ = 1; i_ < lnx - 1; i_++)
+    for (int j_ = 1; j_ < lny - 1; j_++) {
+      double w = d->weight[n][(size_t)i_ * lny + j_];
+      if (w == 0) continue;
+      int i = pd_gx(d, i_, xo) * sc, j = pd_gy(d, j_, yo) * sc;
+      if (!(i >= 0 && j >= 0 && i < d->nx && j < d->ny)) continue;
+      double f00r = sm[(size_t)i_ * lny + j_].re, f00i = sm[(size_t)i_ * lny + j_].im;
+      if (sc != 1) {
+        double f10r = sm[(size_t)(i_ + 1) * lny + j_].re, f10i = sm[(size_t)(i_ + 1) * lny + j_].im;
+        double f01r = sm[(size_t)i_ * lny + j_ + 1].re, f01i = sm[(size_t)i_ * lny + j_ + 1].im;
+        double f11r = sm[(size_t)(i_ + 1) * lny + j_ + 1].re, f11i = sm[(size_t)(i_ + 1) * lny + j_ + 1].im;
+        for (int di = 0; di < sc; di++)
+          for (int dj = 0; dj < sc; dj++) {
+            double x = (di + 0.5 * ((sc + 1) % 2)) / (double)sc, y = (dj + 0.5 * ((sc + 1) % 2)) / (double)sc;
+            double x1 = 1 - x, y1 = 1 - y;
+            double vr = f00r * x1 * y1 + f10r * x * y1 + f01r * x1 * y + f11r * x * y;
+            double vi = f00i * x1 * y1 + f10i * x * y1 + f01i * x1 * y + f11i * x * y;
+            int ni = (int)((i_ + 0.5 - xo - d->x_min) * sc + di), nj = (int)((j_ + 0.5 - yo - d->y_min) * sc + dj);
+            if (ni < 0 || nj < 0 || ni >= d->nx || nj >= d->ny) continue;
+            cplx *o = &d->object[(size_t)ni * d->ny + nj];
+            double nr = w * vr + o->re, nim = w * vi + o->im;
+            if (!d->parallel) { nr -= w * o->re; nim -= w * o->im; }
+            o->re = (real)nr; o->im = (real)nim;
+          }
+      } else {
+        cplx *o = &d->object[(size_t)i * d->ny + j];
+        double nr = w * f00r + o->re, nim = w * f00i + o->im;
+        if (!d->parallel) { nr -= w * o->re; nim -= w * o->im; }
+        o->re = (real)nr; o->im = (real)nim;
+      }
+    }
+}
+
+/* update_from_object: PhaseDiverseCDI.c++:1066-1130.  Sub-pixel blocks
+ * reaching past the object are not read (the reference does not check). */
+static void pd_update(opd *d, int n) {
+  pd_weights(d);
+  double xo = d->xpos[n], yo = d->ypos[n];
+  cplx *sm = d->result[n];
+  int lnx = d->frame[n]->nx, lny = d->frame[n]->ny, sc = d->scale;
+  for (int i_ = 0; i_ < lnx; i_++)
+    for (int j_ = 0; j_ < lny; j_++) {
+      if (d->weight[n][(size_t)i_ * lny + j_] == 0) continue;
+      int i = pd_gx(d, i_, xo) * sc, j = pd_gy(d, j_, yo) * sc;
+      if (!(i >= 0 && j >= 0 && i < d->nx && j < d->ny)) continue;
+      if (sc != 1) {
+        double rv = 0, iv = 0;
+        for (int di = 0; di < sc; di++)
+          for (int dj = 0; dj < sc; dj++)
+            if (i + di < d->nx && j + dj < d->ny) {
+              rv += d->object[(size_t)(i + di) * d->ny + j + dj].re;
+              iv += d->object[(size_t)(i + di) * d->ny + j + dj].im;
+            }
+        double nm = 1 / ((double)(sc * sc));
+        sm[(size_t)i_ * lny + j_].re = (real)(rv * nm);
+        sm[(size_t)i_ * lny + j_].im = (real)(iv * nm);
+      } else {
+        sm[(size_t)i_ * lny + j_] = d->object[(size_t)i * d->ny + j];
+      }
+    }
+  pd_set_result(d, n);
+}
+
+/* scale_object: PhaseDiverseCDI.c++:270-313 */
+static void pd_scale_object(opd *d, double factor) {
+  pd_weights(d);
+  for (int i = 0; i < d->nx; i++)
+    for (int j = 0; j < d->ny; j++) {
+      int not_in = 1;
+      for (int n = 0; not_in && n < d->nframes; n++) {
+        int i_ = pd_lx(d, i / d->scale, d->xpos[n]), j_ = pd_ly(d, j / d->scale, d->ypos[n]);
+        int lnx = d->frame[n]->nx, lny = d->frame[n]->ny;
+        if (i_ >= 0 && j_ >= 0 && i_ < lnx && j_ < lny) not_in = d->weight[n][(size_t)i_ * lny + j_] == 0;
+      }
+      cplx *o = &d->object[(size_t)i * d->ny + j];
+      if (not_in) { o->re = 1; o->im = 0; }
+      else { o->re = (real)(factor * o->re); o->im = (real)(factor * o->im); }
+    }
+}
+
+/* initialise_estimate: PhaseDiverseCDI.c++:182-199 */
+void opd_initialise_estimate(opd *d) {
+  for (size_t p = 0; p < (size_t)d->nx * d->ny; p++) { d->object[p].re = 1; d->object[p].im = 0; }
+  for (int n = 0; n < d->nframes; n++) pd_add(d, n);
+}
+
+/* iterate: PhaseDiverseCDI.c++:202-267 */
+void opd_iterate(opd *d) {
+  for (int n = 0; n < d->nframes; n++) {
+    pd_update(d, n);
+    for (int k = 0; k < d->ipc; k++) ocdi_iterate(d->frame[n]);
+    if (!d->parallel) pd_add(d, n);
+  }
+  if (d->parallel) {
+    pd_scale_object(d, 1 - d->beta);
+    for (int n = 0; n < d->nframes; n++) pd_add(d, n);
+  }
+}
+
+void opd_get_size(opd *d, int *nx, int *ny) { *nx = d->nx; *ny = d->ny; }
+void opd_get_transmission(opd *d, cplx *out) { memcpy(out, d->object, sizeof(cplx) * (size_t)d->nx * d->ny); }
+/* set_transmission: PhaseDiverseCDI.c++:94-113 */
+int opd_set_transmission(opd *d, const cplx *t, int nx, int ny) {
+  if (!d->object) pd_realloc(d, nx, ny);
+  if (nx != d->nx || ny != d->ny) return -1;
+  memcpy(d->object, t, sizeof(cplx) * (size_t)nx * ny);
+  return 0;
+}
+void opd_get_weight(opd *d, int n, real *out) {
+  pd_weights(d);
+  memcpy(out, d->weight[n], sizeof(real) * NPIX(d->frame[n]));
+}
+void opd_get_origin(opd *d, int *x_min, int *y_min) { *x_min = d->x_min; *y_min = d->y_min; }

@@ -81,6 +81,20 @@ def _lib(double=True):
         "otc_get_c_mean": (D, [P, I]),
         "otc_apply": (None, [P, P]),
         "ocdi_set_complex_constraint": (None, [P, P]),
+        "opd_create": (P, [D, D, I, I]),
+        "opd_destroy": (None, [P]),
+        "opd_set_iterations_per_cycle": (None, [P, I]),
+        "opd_set_feedback_parameter": (None, [P, D]),
+        "opd_set_amplification_factor": (None, [P, D]),
+        "opd_set_probe_scaling": (None, [P, I, D]),
+        "opd_add_position": (I, [P, P, D, D, D]),
+        "opd_initialise_estimate": (None, [P]),
+        "opd_iterate": (None, [P]),
+        "opd_get_size": (None, [P, P, P]),
+        "opd_get_transmission": (None, [P, P]),
+        "opd_set_transmission": (I, [P, P, I, I]),
+        "opd_get_weight": (None, [P, I, P]),
+        "opd_get_origin": (None, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -304,3 +318,68 @@ class OracleCDI:
 
     def wf_set_support(self, z_diameter, size=1.01):
         self.lib.ocdi_wf_set_support(self.h, z_diameter, size)
+
+
+class OraclePhaseDiverse:
+    """PhaseDiverseCDI (NADIA/src/PhaseDiverseCDI.c++) over OracleCDI frames."""
+
+    def __init__(self, beta=1.0, gamma=1.0, parallel=False, granularity=1, double=True):
+        self.double = double
+        self.lib = _lib(double)
+        self.cdt = np.complex128 if double else np.complex64
+        self.rdt = np.float64 if double else np.float32
+        self.h = self.lib.opd_create(beta, gamma, int(parallel), int(granularity))
+        self.frames = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.opd_destroy(self.h)
+            self.h = None
+
+    def add_new_position(self, frame, x=0.0, y=0.0, alpha=1.0):
+        self.frames.append(frame)  # the frames must outlive this object
+        return self.lib.opd_add_position(self.h, frame.h, float(x), float(y), float(alpha))
+
+    def set_iterations_per_cycle(self, n):
+        self.lib.opd_set_iterations_per_cycle(self.h, int(n))
+
+    def set_feedback_parameter(self, beta):
+        self.lib.opd_set_feedback_parameter(self.h, float(beta))
+
+    def set_amplification_factor(self, gamma):
+        self.lib.opd_set_amplification_factor(self.h, float(gamma))
+
+    def set_probe_scaling(self, n, alpha):
+        self.lib.opd_set_probe_scaling(self.h, int(n), float(alpha))
+
+    def initialise_estimate(self):
+        self.lib.opd_initialise_estimate(self.h)
+
+    def iterate(self):
+        self.lib.opd_iterate(self.h)
+
+    def size(self):
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        self.lib.opd_get_size(self.h, ctypes.byref(nx), ctypes.byref(ny))
+        return nx.value, ny.value
+
+    def origin(self):
+        x, y = ctypes.c_int(), ctypes.c_int()
+        self.lib.opd_get_origin(self.h, ctypes.byref(x), ctypes.byref(y))
+        return x.value, y.value
+
+    def get_transmission(self):
+        nx, ny = self.size()
+        out = np.zeros((nx, ny), self.cdt)
+        self.lib.opd_get_transmission(self.h, _ptr(out))
+        return out
+
+    def set_transmission(self, t):
+        t = np.ascontiguousarray(t, dtype=self.cdt)
+        return self.lib.opd_set_transmission(self.h, _ptr(t), t.shape[0], t.shape[1])
+
+    def get_weight(self, n):
+        f = self.frames[n]
+        out = np.zeros((f.nx, f.ny), self.rdt)
+        self.lib.opd_get_weight(self.h, int(n), _ptr(out))
+        return out

@@ -1,0 +1,94 @@
+"""PhaseDiverseCDI on the device (nadia_pd) against the oracle: series and
+parallel modes, planar and Fresnel frames, sub-pixel granularity."""
+import numpy as np
+import pytest
+
+from test_gpu_fresnel import fresnel_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def planar_frames(nd, oracle, synth, positions, n=64, alg="ER", double=True):
+    gpu, cpu = [], []
+    for k, _ in enumerate(positions):
+        d = synth.planar_block(n=n, block=24, support=30, seed_amp=10 + k, seed_poisson=10 + k, seed_phase=20 + k)
+        cdt = np.complex128 if double else np.complex64
+        est = nd.Complex2D(n, n, double=double, data=d["x0"].astype(cdt))
+        g = nd.PlanarCDI(est)
+        xo = d["x0"].astype(np.complex128 if double else np.complex64).copy()
+        o = oracle.OracleCDI(xo, oracle.PLANAR, double=double)
+        for obj in (g, o):
+            obj.set_support(d["support"])
+            obj.set_intensity(d["intensity"])
+            obj.set_algorithm(getattr(oracle, alg))
+        gpu.append((est, g))
+        cpu.append((xo, o))
+    return gpu, cpu
+
+
+@pytest.mark.parametrize("parallel", [False, True], ids=["series", "parallel"])
+@pytest.mark.parametrize("scale", [1, 2])
+def test_planar_frames(nd, oracle, synth, parallel, scale):
+    pos = [(0, 0), (12.5, -7.25), (-20, 9)]
+    gpu, cpu = planar_frames(nd, oracle, synth, pos, alg="HIO")
+    pg = nd.PhaseDiverseCDI(beta=0.8, gamma=1.0, parallel=parallel, granularity=scale)
+    po = oracle.OraclePhaseDiverse(beta=0.8, gamma=1.0, parallel=parallel, granularity=scale)
+    for (est, g), (xo, o), (x, y) in zip(gpu, cpu, pos):
+        pg.add_new_position(g, x, y, 1.0)
+        po.add_new_position(o, x, y, 1.0)
+    assert pg.size() == po.size()
+    pg.set_iterations_per_cycle(2)
+    po.set_iterations_per_cycle(2)
+    pg.initialise_estimate()
+    po.initialise_estimate()
+    assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-14
+    for it in range(3):
+        eg = pg.iterate()
+        po.iterate()
+        eo = np.array([o.get_error() for _, o in cpu])
+        assert (np.abs(eg - eo) / eo).max() <= 1e-9, it
+        assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-10, it
+    for (est, g), (xo, o) in zip(gpu, cpu):
+        assert rel(est.get(), xo) <= 1e-10
+
+
+def test_fresnel_frames(nd, oracle, synth):
+    pg = nd.PhaseDiverseCDI(beta=0.5, gamma=1.5, parallel=True)
+    po = oracle.OraclePhaseDiverse(beta=0.5, gamma=1.5, parallel=True)
+    keep = []
+    for k, (x, y) in enumerate([(0, 0), (6, -4)]):
+        d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
+        g.initialise_estimate(k)
+        o.initialise_estimate(k)
+        pg.add_new_position(g, x, y, 1.0 + 0.5 * k)
+        po.add_new_position(o, x, y, 1.0 + 0.5 * k)
+        keep.append((est, g, xo, o))
+    pg.initialise_estimate()
+    po.initialise_estimate()
+    assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-13
+    for it in range(3):
+        pg.iterate()
+        po.iterate()
+    assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-10
+    for est, g, xo, o in keep:
+        assert rel(est.get(), xo) <= 1e-10
+
+
+def test_fp32_frames(nd, oracle, synth):
+    pos = [(0, 0), (8, 8)]
+    gpu, cpu = planar_frames(nd, oracle, synth, pos, alg="ER", double=False)
+    pg = nd.PhaseDiverseCDI(beta=1.0, parallel=False)
+    po = oracle.OraclePhaseDiverse(beta=1.0, parallel=False, double=False)
+    for (est, g), (xo, o), (x, y) in zip(gpu, cpu, pos):
+        pg.add_new_position(g, x, y)
+        po.add_new_position(o, x, y)
+    pg.initialise_estimate()
+    po.initialise_estimate()
+    for _ in range(4):
+        pg.iterate()
+        po.iterate()
+    assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-5

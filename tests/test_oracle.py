@@ -594,3 +594,127 @@ def test_fresnel_support_applies_constraint_to_transmission(oracle):
     tc0 = oracle.OracleConstraint(n, n)  # the same constraint applied by hand
     tc0.add_complex_constraint(np.ones((n, n)), 0.6, 0.6)
     assert rel(e3, (tc0.apply(T2.copy()) - 1) * Ls) < 1e-12
+
+
+# ------------------------------------------------------- PhaseDiverseCDI ----
+# Pinned by structure: the global object is the bounding box of the frames;
+# a single frame with unit weights (series, beta 1) makes the big iteration
+# the frame's own iteration; two identical frames in parallel mode give
+# (1 - beta) object + beta frame; weights are alpha (|L_s|/max)^gamma on the
+# support; 2x sub-pixel sampling maps constants to constants.
+
+def planar_frame(oracle, n=32, seed=0, support=14):
+    rng = np.random.default_rng(seed)
+    obj = np.zeros((n, n), complex)
+    s0 = n // 2 - support // 2
+    obj[s0 + 2:s0 + support - 2, s0 + 2:s0 + support - 2] = rng.uniform(0.5, 1, (support - 4, support - 4))
+    I = np.abs(np.fft.fftshift(np.fft.fft2(obj)) / n) ** 2
+    sup = np.zeros((n, n)); sup[s0:s0 + support, s0:s0 + support] = 1
+    x = (sup * np.exp(1j * rng.uniform(0, 2 * np.pi, (n, n)))).astype(complex)
+    o = oracle.OracleCDI(x, oracle.PLANAR)
+    o.set_support(sup)
+    o.set_intensity(I)
+    o.set_algorithm(oracle.HIO)
+    return o, x, sup, I
+
+
+def test_phase_diverse_geometry(oracle):
+    pd = oracle.OraclePhaseDiverse()
+    a, *_ = planar_frame(oracle, 32, 1)
+    b, *_ = planar_frame(oracle, 32, 2)
+    pd.add_new_position(a, 0, 0)
+    pd.add_new_position(b, -8, 5)
+    # frame at (x, y) covers object pixels local - (x, y) - origin: the box
+    # of [-x, -x + 32) over both frames is [0, 40) x [-5, 32)
+    assert pd.size() == (40, 37) and pd.origin() == (0, -5)
+    pd2 = oracle.OraclePhaseDiverse()
+    pd2.add_new_position(a, 0, 0)
+    pd2.add_new_position(b, 10.5, -4.25)
+    nx, ny = pd2.size()
+    xm, ym = pd2.origin()
+    for f, (x, y) in ((a, (0, 0)), (b, (10.5, -4.25))):
+        for l in (1, 30):   # the interior pixels land inside the object
+            assert 0 <= int(l - x - xm) < nx and 0 <= int(l - y - ym) < ny
+
+
+def test_phase_diverse_single_frame_is_the_frame_iteration(oracle):
+    o, x, sup, I = planar_frame(oracle, 32, 3)
+    ref_x = x.copy()
+    ref = oracle.OracleCDI(ref_x, oracle.PLANAR)
+    ref.set_support(sup)
+    ref.set_intensity(I)
+    ref.set_algorithm(oracle.HIO)
+    pd = oracle.OraclePhaseDiverse(beta=1.0)
+    pd.add_new_position(o, 0, 0)
+    assert np.array_equal(pd.get_weight(0), sup)
+    pd.initialise_estimate()
+    t = pd.get_transmission()
+    inner = sup.astype(bool)
+    # obj + w f - w obj with w = 1: f up to the rounding of (f + obj) - obj
+    assert np.abs(t[inner] - x[inner]).max() < 1e-15 and np.all(t[~inner] == 1)
+    for _ in range(3):
+        pd.iterate()
+        ref.iterate()
+    assert rel(x, ref_x) < 1e-13
+    assert np.abs(pd.get_transmission()[inner] - ref_x[inner]).max() < 1e-13
+
+
+def test_phase_diverse_parallel_mix(oracle):
+    beta = 0.7
+    o1, x1, sup, I = planar_frame(oracle, 32, 4)
+    o2, x2, _, _ = planar_frame(oracle, 32, 4)
+    pd = oracle.OraclePhaseDiverse(beta=beta, parallel=True)
+    pd.add_new_position(o1, 0, 0)
+    pd.add_new_position(o2, 0, 0)
+    assert np.allclose(pd.get_weight(0), sup * beta / 2, rtol=1e-15, atol=0)
+    pd.initialise_estimate()
+    before = pd.get_transmission()
+    pd.iterate()
+    # each frame ran one HIO step from the object; both are identical
+    assert np.array_equal(x1, x2)
+    inner = sup.astype(bool)
+    expect = (1 - beta) * before + beta * x1
+    after = pd.get_transmission()
+    assert np.abs(after[inner] - expect[inner]).max() < 1e-14
+    assert np.all(after[~inner] == 1)
+
+
+def test_phase_diverse_fresnel_weights(oracle):
+    n = 32
+    o, x, wf, rng = fresnel(oracle, n)
+    sup = np.zeros((n, n)); sup[6:26, 8:24] = 1
+    o.set_support(sup)
+    pd = oracle.OraclePhaseDiverse(beta=0.5, gamma=1.7)
+    pd.add_new_position(o, 0, 0, alpha=2.0)
+    m = np.abs(o.get_illumination_at_sample())
+    expect = np.where(sup > 0, 2.0 * (m / m.max()) ** 1.7 * 0.5, 0.0)
+    assert np.abs(pd.get_weight(0) - expect).max() < 1e-14
+
+
+def test_phase_diverse_subpixel_constant(oracle):
+    o, x, sup, I = planar_frame(oracle, 16, 5, support=8)
+    x[:] = (0.25 - 0.5j) * sup
+    pd = oracle.OraclePhaseDiverse(beta=1.0, granularity=2)
+    pd.add_new_position(o, 0, 0)
+    assert pd.size() == (32, 32)
+    pd.initialise_estimate()
+    t = pd.get_transmission()
+    # interior support pixels (whose 2x2 neighbourhood is inside the support)
+    # give their constant to every sub-pixel point of their block
+    s0 = 4
+    for i_ in range(s0 + 1, s0 + 7):
+        for j_ in range(s0 + 1, s0 + 7):
+            blk = t[2 * i_:2 * i_ + 2, 2 * j_:2 * j_ + 2]
+            assert np.abs(blk - (0.25 - 0.5j)).max() < 1e-15
+    # update_from_object averages each block back: a big iteration starts
+    # the frame from the constant
+    pd2 = oracle.OraclePhaseDiverse(beta=1.0, granularity=2)
+    o3, x3, sup3, _ = planar_frame(oracle, 16, 6, support=8)
+    pd2.add_new_position(o3, 0, 0)
+    pd2.initialise_estimate()
+    tt = pd2.get_transmission()
+    tt[:] = 0.5 + 0.25j
+    assert pd2.set_transmission(tt) == 0
+    pd2.set_iterations_per_cycle(0)
+    pd2.iterate()
+    assert np.abs(x3[sup3 > 0] - (0.5 + 0.25j)).max() < 1e-15
