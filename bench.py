@@ -13,6 +13,7 @@ sample.  Further legs (each can be skipped):
   accuracy   final-object error of the FP32 path against the FP64 path for
              the 2048^2 HIO + shrinkwrap + ER schedule, the 64-job batch and
              the 1024^2 Fresnel workload
+  phase_diverse  8 Fresnel frames of 1024^2 merged by PhaseDiverseCDI
 
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -225,6 +226,38 @@ def leg_fresnel(nd, synth, n=1024, wf_iters=50, er_iters=300):
     return out
 
 
+def leg_phase_diverse(nd, synth, n=1024, frames=8, big_iters=10):
+    """PhaseDiverseCDI: `frames` Fresnel frames of n^2 (the 1024^2 Fresnel
+    workload, shifted by 16 px per frame along x) merged in parallel mode,
+    one ER iteration per frame per big iteration, FP32 and FP64."""
+    d = synth.fresnel_config(n=n)
+    out = {"n": n, "frames": frames, "big_iterations": big_iters, "mode": "parallel, beta 0.9, gamma 1"}
+    for tag, double in (("fp32", False), ("fp64", True)):
+        cdt = np.complex128 if double else np.complex64
+        keep = []
+        pd = nd.PhaseDiverseCDI(beta=0.9, gamma=1.0, parallel=True)
+        for k in range(frames):
+            est = nd.Complex2D(n, n, double=double)
+            f = nd.FresnelCDI(est, d["white_field"].astype(cdt), d["wavelength"], d["zfd"], d["zfs"], d["pixel"], 1.0)
+            f.set_support(d["support"])
+            f.set_intensity(d["intensity"])
+            f.initialise_estimate(k)
+            pd.add_new_position(f, 16.0 * k, 0.0)
+            keep.append((est, f))
+        pd.initialise_estimate()
+        pd.iterate()  # warm-up: weights, tile variants
+        nd.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(big_iters):
+            e = pd.iterate()
+        nd.synchronize()
+        dt = time.perf_counter() - t0
+        out[tag] = {"big_iter_s": round(big_iters / dt, 2), "frame_iter_s": round(frames * big_iters / dt, 1),
+                    "object": list(pd.size()), "mean_frame_error": float(np.mean(e))}
+        del pd, keep
+    return out
+
+
 def leg_served(nd, synth, count, rate, budget):
     """Poisson request stream through the continuous-batching loop (FP32
     bulk, FP64 reductions), admission by an HBM budget."""
@@ -270,6 +303,7 @@ def main():
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--no-fresnel", action="store_true", help="skip the 1024^2 Fresnel workload")
     ap.add_argument("--no-served", action="store_true", help="skip the request stream")
+    ap.add_argument("--no-phase-diverse", action="store_true", help="skip the PhaseDiverseCDI leg")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -347,7 +381,10 @@ def main():
         "col_pass_ms": round(avg["col"], 5), "row_pass_ms": round(avg["row"], 5),
         "col_pass_event_ms": round(ev["col"], 5), "row_pass_event_ms": round(ev["row"], 5),
         "tile_variants": list(p.get_tile_variants()),
-        "iteration_achieved_GBs": round(b32["iter"] * N * N * single / 1e9, 1),
+        # the scope's count for an HIO iteration (7 complex + 2 real per pixel)
+        # and the bytes the two fused passes move (6 complex + 2 real)
+        "iteration_effective_GBs": round(b32["iter"] * N * N * single / 1e9, 1),
+        "iteration_moved_GBs": round((b32["col"] + b32["row"]) * N * N * single / 1e9, 1),
     }
     del p, est
     if args.fp32_only:
@@ -376,6 +413,9 @@ def main():
     if not args.no_fresnel:
         extra["fresnel1024"] = leg_fresnel(nd, synth)
         print("bench: fresnel %s" % json.dumps(extra["fresnel1024"]), file=sys.stderr, flush=True)
+    if not args.no_phase_diverse:
+        extra["phase_diverse"] = leg_phase_diverse(nd, synth)
+        print("bench: phase diverse %s" % json.dumps(extra["phase_diverse"]), file=sys.stderr, flush=True)
     if not args.no_served:
         extra["served"] = leg_served(nd, synth, args.serve_requests, args.serve_rate, args.serve_budget_gb * 1e9)
         print("bench: served %s" % json.dumps(extra["served"]), file=sys.stderr, flush=True)
