@@ -278,8 +278,8 @@ int nadia_batch_iterate(nadia_batch* b, int n, double* errors);
 int nadia_batch_iterate_each(nadia_batch* b, const int* counts);
 /* Graph mode: with steps > 0, a nadia_batch_iterate of n >= steps iterations
  * runs as replays of one captured hipGraph holding `steps` iterations of
- * every launch group (groups as parallel branches); the graph is rebuilt
- * only when the batch's launches change.  0 (default) launches eagerly.
+ * every launch group (in launch order); the graph is rebuilt only when the
+ * batch's launches change.  0 (default) launches eagerly.
  * Results are bit-identical to eager launches.  Applies to batches whose
  * jobs all take the fused path and keep no n-best snapshots. */
 int nadia_batch_set_graph(nadia_batch* b, int steps);

@@ -718,3 +718,25 @@ def test_phase_diverse_subpixel_constant(oracle):
     pd2.set_iterations_per_cycle(0)
     pd2.iterate()
     assert np.abs(x3[sup3 > 0] - (0.5 + 0.25j)).max() < 1e-15
+
+
+def test_intensity_autocorrelation_is_object_autocorrelation(oracle):
+    """PlanarCDI::get_intensity_autocorrelation = |IDFT(I)| centred and scaled
+    (the (1+i)/sqrt(2) factor has unit modulus): by Wiener-Khinchin the
+    centred cyclic autocorrelation of the object, computed here by brute force."""
+    n = 16
+    rng = np.random.default_rng(44)
+    obj = np.zeros((n, n))
+    obj[5:11, 4:9] = rng.uniform(0.2, 1.0, (6, 5))
+    I = np.abs(np.fft.fftshift(np.fft.fft2(obj)) / n) ** 2
+    o = oracle.OracleCDI(np.zeros((n, n), complex), oracle.PLANAR)
+    o.set_intensity(I)
+    got = o.get_intensity_autocorrelation()
+    R = np.zeros((n, n))
+    for kx in range(n):
+        for ky in range(n):
+            R[kx, ky] = np.sum(np.roll(np.roll(obj, -kx, 0), -ky, 1) * obj)
+    # I = |DFT obj|^2 / N, so the unnormalised IDFT of I is R; invert(true)
+    # centres it and divides by sqrt(N) = n
+    expect = np.fft.fftshift(np.abs(R)) / n
+    assert np.abs(got - expect).max() <= 1e-12 * expect.max()
