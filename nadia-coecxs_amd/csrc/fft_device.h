@@ -88,6 +88,38 @@ __device__ __forceinline__ void dft(c2_t<T> (&a)[R]) {
     a[2] = csub(t0, t2);
     a[1] = cadd(t1, u);
     a[3] = csub(t1, u);
+  } else if constexpr (R == 16) {
+    // 4 x 4: radix-4 DFTs over n1 for each n2 (input n = 4 n1 + n2), the
+    // twiddles W16^(n2 k1), radix-4 DFTs over n2; output k = k1 + 4 k2
+    const T h = (T)0.70710678118654752440084436210485;  // cos(pi/4)
+    const T c1 = (T)0.92387953251128675612818318939679;  // cos(pi/8)
+    const T s1 = (T)0.38268343236508977172845998403040;  // sin(pi/8)
+    C y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      C b[4] = {a[n2], a[n2 + 4], a[n2 + 8], a[n2 + 12]};
+      dft<4, DIR, T>(b);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = b[k1];
+    }
+    // W16^j = cos(2 pi j / 16) + DIR i sin(2 pi j / 16)
+    const T sd = DIR < 0 ? (T)-1 : (T)1;
+    y[1][1] = cmul(y[1][1], C{c1, sd * s1});   // j = 1
+    y[1][2] = cmul(y[1][2], C{h, sd * h});     // j = 2
+    y[1][3] = cmul(y[1][3], C{s1, sd * c1});   // j = 3
+    y[2][1] = cmul(y[2][1], C{h, sd * h});     // j = 2
+    y[2][2] = cmul_i<DIR>(y[2][2]);            // j = 4
+    y[2][3] = cmul(y[2][3], C{-h, sd * h});    // j = 6
+    y[3][1] = cmul(y[3][1], C{s1, sd * c1});   // j = 3
+    y[3][2] = cmul(y[3][2], C{-h, sd * h});    // j = 6
+    y[3][3] = cmul(y[3][3], C{-c1, -sd * s1}); // j = 9
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      C b[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+      dft<4, DIR, T>(b);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) a[k1 + 4 * k2] = b[k2];
+    }
   } else if constexpr (R == 8) {
     const T h = (T)0.70710678118654752440084436210485;  // sqrt(2)/2
     // even / odd radix-4 halves
