@@ -559,6 +559,52 @@ int write_cplx(std::string file_name, const Complex_2D& complex) {
   return SUCCESS;
 }
 
+// ---- read_image / write_image (NADIA/include/io.h:213-284) -------------------
+// The format comes from a substring of the file name, tested in the
+// reference's order; only .dbin is handled here.
+
+namespace {
+
+bool name_has(const std::string& name, const char* ext) { return name.find(ext) != std::string::npos; }
+
+// the image formats whose readers and writers are not part of this library
+const char* other_format(const std::string& name) {
+  if (name_has(name, ".tiff") || name_has(name, ".tif")) return "TIFF";
+  if (name_has(name, ".ppm")) return "PPM";
+  if (name_has(name, ".txt")) return "text";
+  if (name_has(name, ".hdf")) return "HDF";
+  return 0;
+}
+
+}  // namespace
+
+void read_image(std::string file_name, Double_2D& data, int nx, int ny, const char*) {
+  int status = FAILURE;
+  if (const char* f = other_format(file_name))
+    cout << f << " files are not read by this library (use .dbin)" << endl;
+  if (name_has(file_name, ".dbin")) {
+    if (nx == 0 || ny == 0)
+      cout << "Please pass the dimensions of the dbin file you wish to read to the read_image method." << endl;
+    else
+      status = read_dbin(file_name, nx, ny, data);
+  }
+  if (status == FAILURE) {
+    cout << "Failed to read the file: " << file_name << ". Exiting now.." << endl;
+    exit(0);
+  }
+}
+
+void write_image(std::string file_name, Double_2D& data, bool, double, double) {
+  int status = FAILURE;
+  if (const char* f = other_format(file_name))
+    cout << f << " files are not written by this library (use .dbin)" << endl;
+  if (name_has(file_name, ".dbin")) status = write_dbin(file_name, data);
+  if (status == FAILURE) {
+    cout << "Failed to write to the file: " << file_name << ". Exiting now.." << endl;
+    exit(0);
+  }
+}
+
 // ---- PhaseDiverseCDI (NADIA/src/PhaseDiverseCDI.c++) ------------------------
 
 PhaseDiverseCDI::PhaseDiverseCDI(double beta, double gamma, bool parallel, int granularity)
