@@ -19,7 +19,6 @@ template <typename T> using c2_t = typename Vec2<T>::type;
 // per-slot address registers.  Raw buffers with stride 0: num_records is the
 // size in bytes; accesses beyond it read 0 and write nothing.
 typedef unsigned int nadia_u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned int nadia_u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, unsigned bytes) {
   // 0x00020000: the gfx9-family descriptor word 3 (32-bit data, raw access)
@@ -32,12 +31,22 @@ __device__ __forceinline__ C bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsi
   else return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
 }
 
+// 16-byte elements are stored as two 8-byte halves.  A buffer store of more
+// than 64 bits of data needs a wait state before a VALU instruction may
+// overwrite its data registers; the compiler does not insert it when the
+// store's soffset is an SGPR, and the FP64 column pass then wrote a few
+// lanes' values after their low dword had been replaced (nondeterministic
+// errors of ~1e-9 in one slot of some threads).  64-bit stores are not
+// subject to it.  The second half goes through soffset + 8 so that the two
+// stores are not merged back into one 128-bit store.
 template <typename C>
 __device__ __forceinline__ void bstore(C v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  if constexpr (sizeof(C) == 8)
+  if constexpr (sizeof(C) == 8) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(nadia_u32x2, v), r, voff, soff, 0);
-  else
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(nadia_u32x4, v), r, voff, soff, 0);
+  } else {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(nadia_u32x2, v.x), r, voff, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(nadia_u32x2, v.y), r, voff, soff + 8u, 0);
+  }
 }
 
 template <typename T>
