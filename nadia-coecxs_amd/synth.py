@@ -228,3 +228,45 @@ def fresnel_config(n=1024, wavelength=1.55e-10, zfs=1.6e-3, zfd=0.909, pixel=13.
     return dict(intensity=I, support=sup, white_field=Ld, white_field_intensity=wf_int, pupil=pupil,
                 transmission=T, esw=esw, wavelength=wavelength, zfd=zfd, zfs=zfs, pixel=pixel, z12=z12,
                 z23=z23, pupil_diameter=pupil_diameter)
+
+
+def disk_field(n, count, rng, radius=(2.0, 5.0)):
+    """n x n binary field: 1 inside any of `count` disks (radius uniform in
+    `radius` px, uniform real centres), 0 elsewhere -- isolated features of one
+    value on an exactly zero background."""
+    yy, xx = np.mgrid[0:n, 0:n]
+    g = np.zeros((n, n))
+    for _ in range(count):
+        cx, cy = rng.uniform(0, n, 2)
+        r = rng.uniform(*radius)
+        g[(xx - cx) ** 2 + (yy - cy) ** 2 < r * r] = 1.0
+    return g
+
+
+def ptycho_frames(m=64, crops=((0, 0), (14, 9), (27, 20)), seed=7):
+    """Overlapping planar frames cut from one complex object, for position
+    refinement.
+
+    Object: (m + max crop) square, magnitude 1.05 - 0.65 g with g a field of
+    disks (disk_field: absorbing features of one value), phase 0.8 g.  The
+    cross-correlation alignment looks at 1 - |T| where |T| < 1 and weights
+    exactly those pixels: a background magnitude clear of 1 keeps that set
+    independent of rounding and leaves only the features, on which the
+    correlation ratio peaks at the true offset.  Frame k is the m x m crop at crops[k] = (X, Y), i.e. frame
+    pixel (i, j) is object pixel (i + X, j + Y); in PhaseDiverseCDI
+    coordinates its position is (-X, -Y).  Each frame comes with the exact
+    noise-free intensity |F frame|^2 (unitary centred transform) and a full
+    support, so the frame is a fixed point of its own ER / HIO iteration.
+    Returns dict(object, frames, intensities, supports, positions).
+    """
+    rng = np.random.default_rng(seed)
+    big = m + max(max(c) for c in crops)
+    g = disk_field(big, big // 4, rng)
+    obj = (1.05 - 0.65 * g) * np.exp(0.8j * g)
+    frames, ints = [], []
+    for X, Y in crops:
+        f = obj[X:X + m, Y:Y + m].copy()
+        frames.append(f)
+        ints.append(far_field(f))
+    return dict(object=obj, frames=frames, intensities=ints, supports=[np.ones((m, m)) for _ in crops],
+                positions=[(-float(X), -float(Y)) for X, Y in crops])

@@ -1191,3 +1191,209 @@ void opd_get_weight(opd *d, int n, real *out) {
   memcpy(out, d->weight[n], sizeof(real) * NPIX(d->frame[n]));
 }
 void opd_get_origin(opd *d, int *x_min, int *y_min) { *x_min = d->x_min; *y_min = d->y_min; }
+
+/* ------------------------------------------------------------------------ */
+/* Position refinement: PhaseDiverseCDI::adjust_positions / check_position  */
+/* (NADIA/src/PhaseDiverseCDI.c++:337-719) with align and interpolate       */
+/* (NADIA/src/utils.c:1151-1435, defaults NADIA/include/utils.h:24-30).      */
+/* ------------------------------------------------------------------------ */
+
+/* align: the weighted cross-correlation of two real images zero-padded to
+ * twice the larger size.  FFTW's r2c / c2r pair becomes complex transforms
+ * of the real arrays (same products; the imaginary parts are zero to
+ * rounding and dropped).  Returns 1 when an offset was found. */
+int oalign(const real *a, int nx1, int ny1, const real *b, int nx2, int ny2, int *ox, int *oy, int min_x,
+           int max_x, int min_y, int max_y, const real *wa, const real *wb, double overlap_fraction) {
+  int nx = 2 * (nx1 > nx2 ? nx1 : nx2), ny = 2 * (ny1 > ny2 ? ny1 : ny2);
+  if (min_x == max_x || min_y == max_y) {
+    min_x = (int)(1 - nx / 2.0); max_x = (int)(nx / 2.0);
+    min_y = (int)(1 - ny / 2.0); max_y = (int)(ny / 2.0);
+  }
+  size_t np = (size_t)nx * ny;
+  cplx *i1 = calloc_c(np), *i2 = calloc_c(np), *w1 = calloc_c(np), *w2 = calloc_c(np);
+  double weight_sum = 0;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      if (!(i >= 0.5 * (nx - nx1) && i < 0.5 * (nx + nx1) && j >= 0.5 * (ny - ny1) && j < 0.5 * (ny + ny1))) continue;
+      int i_ = (int)(i - 0.5 * (nx - nx1)), j_ = (int)(j - 0.5 * (ny - ny1));
+      size_t p = (size_t)i * ny + j;
+      if (!wa || wa[(size_t)i_ * ny1 + j_] > 0) { i1[p].re = a[(size_t)i_ * ny1 + j_]; w1[p].re = 1; }
+      if (i_ < nx2 && j_ < ny2 && (!wb || wb[(size_t)i_ * ny2 + j_] > 0)) {
+        i2[p].re = b[(size_t)i_ * ny2 + j_];
+        w2[p].re = 1;
+      }
+      weight_sum += (nx1 * ny1 < nx2 * ny2) ? w1[p].re : w2[p].re;
+    }
+  or_fft2d(i1, nx, ny, -1); or_fft2d(i2, nx, ny, -1);
+  or_fft2d(w1, nx, ny, -1); or_fft2d(w2, nx, ny, -1);
+  for (size_t p = 0; p < np; p++) {
+    /* conj(F1) * F2 */
+    real ar = i1[p].re, ai = -i1[p].im, br = i2[p].re, bi = i2[p].im;
+    i1[p].re = ar * br - ai * bi; i1[p].im = ar * bi + ai * br;
+    ar = w1[p].re; ai = -w1[p].im; br = w2[p].re; bi = w2[p].im;
+    w1[p].re = ar * br - ai * bi; w1[p].im = ar * bi + ai * br;
+  }
+  or_fft2d(i1, nx, ny, +1); or_fft2d(w1, nx, ny, +1);
+  double overlap_factor = weight_sum * nx * ny;
+  real best = 0;
+  int found = 0;
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j < ny; j++) {
+      size_t p = (size_t)i * ny + j;
+      real ci = i1[p].re, cw = w1[p].re;
+      if (!(cw / overlap_factor > overlap_fraction)) continue;
+      real ratio = ci / cw;
+      if (!(ratio > best)) continue;
+      int ti = -i, tj = -j;
+      if (i >= nx / 2.0) ti = -i + nx;
+      if (j >= ny / 2.0) tj = -j + ny;
+      if (ti >= min_x && ti < max_x && tj >= min_y && tj < max_y) { best = ratio; *ox = ti; *oy = tj; found = 1; }
+    }
+  free(i1); free(i2); free(w1); free(w2);
+  return found;
+}
+
+/* interpolate(Double_2D): bilinear up-sampling by big/original; the outer
+ * ring of original pixels is not written (NADIA/src/utils.c:1346-1435). */
+void ointerpolate(const real *o, int nx, int ny, real *big, int bnx, int bny) {
+  int sx = bnx / nx, sy = bny / ny;
+  memset(big, 0, sizeof(real) * (size_t)bnx * bny);
+  for (int i = 1; i < nx - 1; i++)
+    for (int j = 1; j < ny - 1; j++) {
+      double f00 = o[(size_t)i * ny + j], f10 = o[(size_t)(i + 1) * ny + j];
+      double f01 = o[(size_t)i * ny + j + 1], f11 = o[(size_t)(i + 1) * ny + j + 1];
+      double ni = (i + 0.5) * sx, nj = (j + 0.5) * sy;
+      for (int di = 0; di < sx; di++)
+        for (int dj = 0; dj < sy; dj++) {
+          double x = (di + 0.5 * ((sx + 1) % 2)) / (double)sx, y = (dj + 0.5 * ((sy + 1) % 2)) / (double)sy;
+          double v = f00 * (1 - x) * (1 - y) + f10 * x * (1 - y) + f01 * (1 - x) * y + f11 * x * y;
+          int bi = (int)(ni + di), bj = (int)(nj + dj);
+          if (bi < bnx && bj < bny) big[(size_t)bi * bny + bj] = (real)v;
+        }
+    }
+}
+
+/* check_position: PhaseDiverseCDI.c++:582-719 (recursive 3 x 3 search with
+ * the step halved when the centre wins; gives up after 10 moves). */
+static int pd_check_position(opd *d, int n, double step, int min_x, int max_x, int min_y, int max_y, int tries,
+                             int *gave_up) {
+  double x = d->xpos[n], y = d->ypos[n];
+  if (step < 1.0 / d->scale) return 1;
+  if (tries > 10 || x < x + min_x || x > x + max_x || y < y + min_y || y > y + max_y) {
+    if (gave_up) *gave_up = 1;
+    return 0;
+  }
+  ocdi *single = d->frame[n];
+  double best_x = x, best_y = y, best_error = 100;
+  size_t no = (size_t)d->nx * d->ny, ns = NPIX(single);
+  cplx *obj_copy = calloc_c(no), *single_copy = calloc_c(ns);
+  memcpy(obj_copy, d->object, sizeof(cplx) * no);
+  memcpy(single_copy, d->result[n], sizeof(cplx) * ns);
+  d->beta = 0.5;
+  d->weights_set = 0;
+  for (int i = -1; i < 2; i++)
+    for (int j = -1; j < 2; j++) {
+      double nxp = x + i * step, nyp = y + j * step;
+      d->xpos[n] = nxp; d->ypos[n] = nyp;
+      pd_add(d, n);
+      pd_update(d, n);
+      ocdi_iterate(single);
+      if (single->current_error < best_error) { best_error = single->current_error; best_x = nxp; best_y = nyp; }
+      memcpy(d->object, obj_copy, sizeof(cplx) * no);
+      memcpy(d->result[n], single_copy, sizeof(cplx) * ns);
+      pd_set_result(d, n);
+    }
+  free(obj_copy); free(single_copy);
+  d->xpos[n] = best_x; d->ypos[n] = best_y;
+  if (best_x == x && best_y == y) step = step / 2.0;
+  if (!pd_check_position(d, n, step, min_x, max_x, min_y, max_y, tries + 1, gave_up)) {
+    d->xpos[n] = x; d->ypos[n] = y;
+    return 0;
+  }
+  d->beta = 1.0;
+  d->weights_set = 0;
+  return 1;
+}
+
+/* adjust_positions: PhaseDiverseCDI.c++:337-579.  type 0 = cross-correlation,
+ * 1 = minimum error.  gave_up[n] (optional) is set when the minimum-error
+ * search gave up on frame n.  Returns -1 for an unknown type or frames of
+ * different sizes. */
+int opd_adjust_positions(opd *d, int type, int forward, int min_x, int max_x, int min_y, int max_y, double step,
+                         int *gave_up) {
+  if (type != 0 && type != 1) return -1;
+  int nf = d->nframes;
+  if (nf == 0 || !d->object) return -1;
+  for (int k = 1; k < nf; k++)
+    if (d->frame[k]->nx != d->frame[0]->nx || d->frame[k]->ny != d->frame[0]->ny) return -1;
+  int parallel_c = d->parallel, nx_c = d->nx, ny_c = d->ny, xmin_c = d->x_min, ymin_c = d->y_min;
+  double beta_c = d->beta;
+  cplx *obj_c = d->object;
+  d->parallel = 0;
+  int n = 1, limit = nf;
+  if (!forward) { n = nf - 2; limit = -1; }
+  cplx *temp_object = calloc_c((size_t)d->nx * d->ny);
+  for (size_t p = 0; p < (size_t)d->nx * d->ny; p++) { temp_object[p].re = 1; temp_object[p].im = 0; }
+  d->object = temp_object;
+  d->beta = 1.0;
+  d->weights_set = 0;
+  int first = forward ? 0 : nf - 1;
+  ocdi_iterate(d->frame[first]);
+  pd_get_result(d, first);
+  pd_add(d, first);
+  while (n != limit) {
+    double bx = d->xpos[n], by = d->ypos[n];
+    int lnx = d->frame[0]->nx, lny = d->frame[0]->ny, sc = d->scale;
+    int mx = lnx * sc, my = lny * sc;
+    ocdi_iterate(d->frame[n]);
+    pd_get_result(d, n);
+    if (type == 0) {
+      real *tsingle = calloc_r((size_t)lnx * lny), *tothers = calloc_r((size_t)mx * my);
+      for (int i = 0; i < mx; i++)
+        for (int j = 0; j < my; j++) {
+          int i_ = (int)(i - bx * sc - d->x_min * sc), j_ = (int)(j - by * sc - d->y_min * sc);
+          if (i_ >= 0 && j_ >= 0 && i_ < d->nx && j_ < d->ny) {
+            double m = c_mag(d->object[(size_t)i_ * d->ny + j_]);
+            if (m < 1) tothers[(size_t)i * my + j] = (real)(1 - m);
+          }
+          if (i < lnx && j < lny) {
+            double m = c_mag(d->result[n][(size_t)i * lny + j]);
+            if (m < 1) tsingle[(size_t)i * lny + j] = (real)(1 - m);
+          }
+        }
+      real *img = tsingle, *wgt = d->frame[n]->support, *ibig = NULL, *wbig = NULL;
+      if (sc != 1) {
+        ibig = calloc_r((size_t)mx * my); wbig = calloc_r((size_t)mx * my);
+        ointerpolate(tsingle, lnx, lny, ibig, mx, my);
+        ointerpolate(d->frame[n]->support, lnx, lny, wbig, mx, my);
+        img = ibig; wgt = wbig;
+      }
+      int new_x = 0, new_y = 0;
+      oalign(img, mx, my, tothers, mx, my, &new_x, &new_y, min_x, max_x, min_y, max_y, wgt, tothers, 0.2);
+      d->xpos[n] = bx + new_x / (double)sc;
+      d->ypos[n] = by + new_y / (double)sc;
+      free(tsingle); free(tothers); free(ibig); free(wbig);
+    } else {
+      cplx *temp = calloc_c((size_t)mx * my);
+      for (int i = 0; i < mx; i++)
+        for (int j = 0; j < my; j++) {
+          int i_ = (int)(i - bx * sc - d->x_min * sc), j_ = (int)(j - by * sc - d->y_min * sc);
+          if (i_ >= 0 && j_ >= 0 && i_ < d->nx && j_ < d->ny) temp[(size_t)i * my + j] = d->object[(size_t)i_ * d->ny + j_];
+        }
+      d->object = temp; d->nx = mx; d->ny = my;
+      d->x_min = (int)(-d->xpos[n]); d->y_min = (int)(-d->ypos[n]);
+      pd_check_position(d, n, step, min_x, max_x, min_y, max_y, 0, gave_up ? &gave_up[n] : NULL);
+      d->object = temp_object; d->nx = nx_c; d->ny = ny_c; d->x_min = xmin_c; d->y_min = ymin_c;
+      free(temp);
+    }
+    d->weights_set = 0;
+    pd_add(d, n);
+    n += forward ? 1 : -1;
+  }
+  d->object = obj_c; d->nx = nx_c; d->ny = ny_c; d->x_min = xmin_c; d->y_min = ymin_c;
+  d->parallel = parallel_c; d->beta = beta_c; d->weights_set = 0;
+  free(temp_object);
+  return 0;
+}
+
+void opd_get_position(opd *d, int n, double *x, double *y) { *x = d->xpos[n]; *y = d->ypos[n]; }

@@ -95,6 +95,10 @@ def _lib(double=True):
         "opd_set_transmission": (I, [P, P, I, I]),
         "opd_get_weight": (None, [P, I, P]),
         "opd_get_origin": (None, [P, P, P]),
+        "opd_adjust_positions": (I, [P, I, I, I, I, I, I, D, P]),
+        "opd_get_position": (None, [P, I, P, P]),
+        "oalign": (I, [P, I, I, P, I, I, P, P, I, I, I, I, P, P, D]),
+        "ointerpolate": (None, [P, I, I, P, I, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -383,3 +387,44 @@ class OraclePhaseDiverse:
         out = np.zeros((f.nx, f.ny), self.rdt)
         self.lib.opd_get_weight(self.h, int(n), _ptr(out))
         return out
+
+    CROSS_CORRELATION, MINIMUM_ERROR = 0, 1
+
+    def adjust_positions(self, type=0, forward=True, x_min=-50, x_max=50, y_min=-50, y_max=50, step_size=4.0):
+        """Returns the per-frame 'gave up' flags of the minimum-error search."""
+        gave_up = np.zeros(len(self.frames), np.int32)
+        rc = self.lib.opd_adjust_positions(self.h, int(type), int(bool(forward)), int(x_min), int(x_max), int(y_min),
+                                           int(y_max), float(step_size), _ptr(gave_up))
+        if rc != 0:
+            raise ValueError("adjust_positions: unknown type or frames of different sizes")
+        return gave_up
+
+    def get_position(self, n):
+        x, y = ctypes.c_double(), ctypes.c_double()
+        self.lib.opd_get_position(self.h, int(n), ctypes.byref(x), ctypes.byref(y))
+        return x.value, y.value
+
+
+def oracle_align(a, b, min_x=0, max_x=0, min_y=0, max_y=0, wa=None, wb=None, overlap_fraction=0.2, double=True):
+    """align (NADIA/src/utils.c:1151-1343): returns (found, offset_x, offset_y)."""
+    lib = _lib(double)
+    rdt = np.float64 if double else np.float32
+    a = np.ascontiguousarray(a, rdt)
+    b = np.ascontiguousarray(b, rdt)
+    wa = None if wa is None else np.ascontiguousarray(wa, rdt)
+    wb = None if wb is None else np.ascontiguousarray(wb, rdt)
+    ox, oy = ctypes.c_int(0), ctypes.c_int(0)
+    found = lib.oalign(_ptr(a), a.shape[0], a.shape[1], _ptr(b), b.shape[0], b.shape[1], ctypes.byref(ox),
+                       ctypes.byref(oy), int(min_x), int(max_x), int(min_y), int(max_y),
+                       None if wa is None else _ptr(wa), None if wb is None else _ptr(wb), float(overlap_fraction))
+    return bool(found), ox.value, oy.value
+
+
+def oracle_interpolate(a, scale, double=True):
+    """interpolate(Double_2D) (NADIA/src/utils.c:1346-1435) by an integer factor."""
+    lib = _lib(double)
+    rdt = np.float64 if double else np.float32
+    a = np.ascontiguousarray(a, rdt)
+    out = np.zeros((a.shape[0] * scale, a.shape[1] * scale), rdt)
+    lib.ointerpolate(_ptr(a), a.shape[0], a.shape[1], _ptr(out), out.shape[0], out.shape[1])
+    return out
