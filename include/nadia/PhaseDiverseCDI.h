@@ -8,8 +8,9 @@
 // series or parallel mode, optionally on a sub-pixel grid (granularity).
 // The global object, the frames' results and weights live on the device
 // (nadia_pd, include/nadia_amd.h); in parallel mode the frames' iterations
-// run as one batch.  Position refinement (adjust_positions) is not
-// provided: it prints a message and leaves the positions unchanged.
+// run as one batch.  adjust_positions (cross-correlation or minimum-error
+// refinement of the frame offsets) runs on the device and prints the
+// reference's per-frame messages.
 #ifndef NADIA_AMD_PHASED_H
 #define NADIA_AMD_PHASED_H
 

@@ -233,8 +233,7 @@ int nadia_recon_set_complex_constraint(nadia_recon* r, nadia_constraint* tc);
  * initialise_estimate; iterate = one "big" iteration (frame_errors, may be
  * NULL, receives each frame's get_error()); get/set_transmission = the
  * global object (nx x ny, interleaved complex, x-major).  In parallel mode
- * the frames' iterations run as one batch.  Position refinement
- * (adjust_positions) is not provided. */
+ * the frames' iterations run as one batch. */
 int nadia_pd_create(double beta, double gamma, int parallel, int granularity, nadia_pd** out);
 int nadia_pd_destroy(nadia_pd* d);
 int nadia_pd_add_position(nadia_pd* d, nadia_recon* frame, double x, double y, double alpha);
@@ -246,6 +245,27 @@ int nadia_pd_iterate(nadia_pd* d, double* frame_errors);
 int nadia_pd_get_size(const nadia_pd* d, int* nx, int* ny);
 int nadia_pd_get_transmission(nadia_pd* d, void* host);
 int nadia_pd_set_transmission(nadia_pd* d, int nx, int ny, const void* host);
+/* adjust_positions (NADIA/include/PhaseDiverseCDI.h:325-329,
+ * NADIA/src/PhaseDiverseCDI.c++:337-719): type 0 = CROSS_CORRELATION,
+ * 1 = MINIMUM_ERROR; forward = align each frame to the previous one (else to
+ * the next, from the last); the search window [x_min, x_max) x [y_min,
+ * y_max) and the initial step of the minimum-error search.  gave_up (may be
+ * NULL, one int per frame) is set where the minimum-error search gave up and
+ * restored the frame's position.  Frames must share one size.  Runs on a
+ * temporary object of ones; the global object is not changed. */
+int nadia_pd_adjust_positions(nadia_pd* d, int type, int forward, int x_min, int x_max, int y_min, int y_max,
+                              double step_size, int* gave_up);
+/* get_final_x_position / get_final_y_position (PhaseDiverseCDI.h:339-352) */
+int nadia_pd_get_position(const nadia_pd* d, int n, double* x, double* y);
+/* align (NADIA/include/utils.h:24-30, NADIA/src/utils.c:1151-1343): the
+ * offset of img2 against img1 (real arrays, x-major, element type of
+ * `precision`) maximising their weighted cross-correlation ratio; w1 / w2
+ * (may be NULL) select the pixels taken (> 0).  min == max on either axis
+ * searches every offset.  *found = 0 (no offset passed the overlap test)
+ * leaves the offsets unchanged. */
+int nadia_align(int precision, const void* img1, int nx1, int ny1, const void* img2, int nx2, int ny2, int min_x,
+                int max_x, int min_y, int max_y, const void* w1, const void* w2, double overlap_fraction,
+                int* offset_x, int* offset_y, int* found);
 
 /* Device memory (bytes) a reconstruction holds besides its estimate field,
  * and an estimate for one not yet created, estimate field included (fused

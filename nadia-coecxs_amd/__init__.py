@@ -132,6 +132,9 @@ def _sig(lib):
         "nadia_pd_get_size": (I, [P, IP, IP]),
         "nadia_pd_get_transmission": (I, [P, P]),
         "nadia_pd_set_transmission": (I, [P, I, I, P]),
+        "nadia_pd_adjust_positions": (I, [P, I, I, I, I, I, I, D, IP]),
+        "nadia_pd_get_position": (I, [P, I, DP, DP]),
+        "nadia_align": (I, [I, P, I, I, P, I, I, I, I, I, I, P, P, D, IP, IP, IP]),
         "nadia_recon_bytes_estimate": (I, [I, I, I, I, I, ctypes.POINTER(ctypes.c_size_t)]),
         "nadia_recon_set_tile_variants": (I, [P, I, I]),
         "nadia_recon_get_tile_variants": (I, [P, IP, IP]),
@@ -536,6 +539,44 @@ class PhaseDiverseCDI:
     def set_transmission(self, t):
         t = np.ascontiguousarray(t, dtype=self.frames[0].cdt)
         _check(library().nadia_pd_set_transmission(self.h, t.shape[0], t.shape[1], _ptr(t)))
+
+    CROSS_CORRELATION, MINIMUM_ERROR = 0, 1
+
+    def adjust_positions(self, type=0, forward=True, x_min=-50, x_max=50, y_min=-50, y_max=50, step_size=4.0):
+        """adjust_positions (NADIA/src/PhaseDiverseCDI.c++:337-579); returns the
+        per-frame flags of frames whose minimum-error search gave up."""
+        gave_up = np.zeros(len(self.frames), np.int32)
+        _check(library().nadia_pd_adjust_positions(self.h, int(type), int(bool(forward)), int(x_min), int(x_max),
+                                                   int(y_min), int(y_max), float(step_size),
+                                                   gave_up.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return gave_up
+
+    def get_position(self, n):
+        x, y = ctypes.c_double(), ctypes.c_double()
+        _check(library().nadia_pd_get_position(self.h, int(n), ctypes.byref(x), ctypes.byref(y)))
+        return x.value, y.value
+
+    def get_final_x_position(self, n):
+        return self.get_position(n)[0]
+
+    def get_final_y_position(self, n):
+        return self.get_position(n)[1]
+
+
+def align(img1, img2, min_x=0, max_x=0, min_y=0, max_y=0, w1=None, w2=None, overlap_fraction=0.2, double=True):
+    """align (NADIA/src/utils.c:1151-1343) on the device: (found, offset_x,
+    offset_y) of img2 against img1."""
+    rdt = np.float64 if double else np.float32
+    a = np.ascontiguousarray(img1, rdt)
+    b = np.ascontiguousarray(img2, rdt)
+    wa = None if w1 is None else np.ascontiguousarray(w1, rdt)
+    wb = None if w2 is None else np.ascontiguousarray(w2, rdt)
+    ox, oy, found = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _check(library().nadia_align(FP64 if double else FP32, _ptr(a), a.shape[0], a.shape[1], _ptr(b), b.shape[0],
+                                 b.shape[1], int(min_x), int(max_x), int(min_y), int(max_y),
+                                 None if wa is None else _ptr(wa), None if wb is None else _ptr(wb),
+                                 float(overlap_fraction), ctypes.byref(ox), ctypes.byref(oy), ctypes.byref(found)))
+    return bool(found.value), ox.value, oy.value
 
 
 CUSTOM_CONSTRAINT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -700,7 +700,31 @@ void PhaseDiverseCDI::set_transmission(Complex_2D& t) {
     nadia_cxx_check(rc, "set_transmission");
 }
 
-void PhaseDiverseCDI::adjust_positions(int, bool, int, int, int, int, double) {
-  cout << "PhaseDiverseCDI::adjust_positions: position refinement is not available in this build; "
-       << "the positions are unchanged" << endl;
+// adjust_positions (NADIA/src/PhaseDiverseCDI.c++:337-579)
+void PhaseDiverseCDI::adjust_positions(int type, bool forwards, int x_min, int x_max, int y_min, int y_max,
+                                       double step_size) {
+  if (type != CROSS_CORRELATION && type != MINIMUM_ERROR) {
+    std::cerr << "The alignment type given to PhaseDiverseCDI::adjust_positions is not a known type. Please "
+              << "consider using PhaseDiverseCDI::CROSS_CORRELATION or  PhaseDiverseCDI::MINIMUM_ERROR." << endl;
+    return;
+  }
+  const int nf = (int)singleCDI.size();
+  if (nf == 0) return;
+  for (int i = 0; i < nf; i++) singleCDI[i]->device_sync();
+  std::vector<int> gave_up(nf, 0);
+  nadia_cxx_check(nadia_pd_adjust_positions(h, type, forwards ? 1 : 0, x_min, x_max, y_min, y_max, step_size,
+                                            gave_up.data()),
+                  "adjust_positions");
+  frames_changed();
+  // the reference reports each aligned frame as it goes: all but the first
+  // (forwards) or the last (backwards), in alignment order
+  for (int k = 1; k < nf; k++) {
+    const int n = forwards ? k : nf - 1 - k;
+    const double before_x = x_position[n], before_y = y_position[n];
+    nadia_cxx_check(nadia_pd_get_position(h, n, &x_position[n], &y_position[n]), "adjust_positions");
+    if (gave_up[n])
+      cout << "Giving up on probe " << n << ". Could not find it's position. Returning to the original. " << endl;
+    cout << "moving prob " << n << " from (" << before_x << "," << before_y << ") -> (" << x_position[n] << ","
+         << y_position[n] << ")." << endl;
+  }
 }

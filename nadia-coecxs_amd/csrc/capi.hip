@@ -2115,6 +2115,242 @@ int pd_scale(nadia_pd* d, double factor) {
   return 0;
 }
 
+// ---- position refinement (PhaseDiverseCDI.c++:337-719, utils.c:1151-1435) --
+
+// Device buffers of one adjust_positions / align call, freed on scope exit.
+struct DevScratch {
+  std::vector<void*> p;
+  DeviceCtx* c;
+  explicit DevScratch(DeviceCtx* ctx) : c(ctx) {}
+  int get(void** out, size_t bytes) {
+    int rc;
+    if ((rc = alloc(out, bytes ? bytes : 1))) return rc;
+    p.push_back(*out);
+    HIPCHK(hipMemsetAsync(*out, 0, bytes ? bytes : 1, c->stream));
+    return 0;
+  }
+  ~DevScratch() {
+    if (p.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (void* q : p) (void)hipFree(q);
+  }
+};
+
+// align (utils.c:1151-1343) on device arrays of element type T: the images
+// and weights zero-padded to twice the larger size, their correlations by
+// 2-D DFTs (complex transforms of the real arrays), then the best ratio.
+// *found = 0 leaves *ox, *oy unchanged.
+template <typename T>
+int align_T(DeviceCtx* c, const void* img1, int nx1, int ny1, const void* img2, int nx2, int ny2, int* ox, int* oy,
+            int* found, int min_x, int max_x, int min_y, int max_y, const void* w1, const void* w2,
+            double overlap_fraction) {
+  const int prec = sizeof(T) == 8 ? NADIA_FP64 : NADIA_FP32;
+  const int nx = 2 * (nx1 > nx2 ? nx1 : nx2), ny = 2 * (ny1 > ny2 ? ny1 : ny2);
+  int rc;
+  if ((rc = check_dims(nx, ny))) return rc;
+  if (min_x == max_x || min_y == max_y) {
+    min_x = (int)(1 - nx / 2.0);
+    max_x = (int)(nx / 2.0);
+    min_y = (int)(1 - ny / 2.0);
+    max_y = (int)(ny / 2.0);
+  }
+  const size_t np = (size_t)nx * ny, cb = np * sizeof(T) * 2;
+  DevScratch sc(c);
+  void *i1, *i2, *c1, *c2, *cnt, *part, *best;
+  if ((rc = sc.get(&i1, cb)) || (rc = sc.get(&i2, cb)) || (rc = sc.get(&c1, cb)) || (rc = sc.get(&c2, cb)) ||
+      (rc = sc.get(&cnt, sizeof(unsigned))) || (rc = sc.get(&part, sizeof(XcBest) * kEwBlocks)) ||
+      (rc = sc.get(&best, sizeof(XcBest))))
+    return rc;
+  XcPlace pl{img1, w1, img2, w2, nx1, ny1, nx2, ny2, i1, i2, c1, c2, nx, ny, (unsigned*)cnt};
+  k_xc_place<T>(pl, c->stream);
+  LAUNCHCHK();
+  for (void* b : {i1, i2, c1, c2})
+    if ((rc = fft2d(c, b, nx, ny, prec, -1))) return rc;
+  k_xc_mulconj<T>(i1, i2, (long long)np, c->stream);
+  k_xc_mulconj<T>(c1, c2, (long long)np, c->stream);
+  LAUNCHCHK();
+  if ((rc = fft2d(c, i1, nx, ny, prec, +1)) || (rc = fft2d(c, c1, nx, ny, prec, +1))) return rc;
+  unsigned count = 0;
+  HIPCHK(hipMemcpyAsync(&count, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  XcScan xs{i1, c1, nx, ny, (double)count * nx * ny, overlap_fraction, min_x, max_x, min_y, max_y};
+  k_xc_scan<T>(xs, (XcBest*)part, c->stream);
+  k_xc_final((const XcBest*)part, kEwBlocks, (XcBest*)best, c->stream);
+  LAUNCHCHK();
+  XcBest hb{};
+  HIPCHK(hipMemcpyAsync(&hb, best, sizeof(XcBest), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *found = hb.index >= 0;
+  if (*found) {
+    const int i = (int)(hb.index / ny), j = (int)(hb.index % ny);
+    *ox = (i >= nx / 2.0) ? -i + nx : -i;
+    *oy = (j >= ny / 2.0) ? -j + ny : -j;
+  }
+  return 0;
+}
+
+int pd_iterate_frame(nadia_pd* d, int n) { return nadia_recon_iterate(d->frames[n].r, 1, nullptr); }
+
+// check_position (PhaseDiverseCDI.c++:582-719): the 3 x 3 positions around
+// the current one, each tried with one iteration of the frame from the
+// object merged at beta 0.5; the position of least error wins, the step is
+// halved when the centre wins; more than 10 moves give up and restore.
+int pd_check_position(nadia_pd* d, int n, double step, int min_x, int max_x, int min_y, int max_y, int tries,
+                      int* gave_up, int* status) {
+  PdFrame& f = d->frames[n];
+  const double x = f.x, y = f.y;
+  if (step < 1.0 / d->scale) {
+    *status = 1;
+    return 0;
+  }
+  if (tries > 10 || x < x + min_x || x > x + max_x || y < y + min_y || y > y + max_y) {
+    if (gave_up) *gave_up = 1;
+    *status = 0;
+    return 0;
+  }
+  DeviceCtx* c = dev_ctx(d->dev);
+  const size_t esz = d->prec == NADIA_FP64 ? 16 : 8;
+  const size_t ob = (size_t)d->nx * d->ny * esz, fb = (size_t)f.r->npix * f.r->esz;
+  DevScratch sc(c);
+  void *obj_copy, *single_copy;
+  int rc;
+  if ((rc = sc.get(&obj_copy, ob)) || (rc = sc.get(&single_copy, fb))) return rc;
+  HIPCHK(hipMemcpyAsync(obj_copy, d->object, ob, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(single_copy, f.result, fb, hipMemcpyDeviceToDevice, c->stream));
+  double best_x = x, best_y = y, best_error = 100;
+  d->beta = 0.5;
+  d->weights_set = false;
+  for (int i = -1; i < 2; i++)
+    for (int j = -1; j < 2; j++) {
+      const double nxp = x + i * step, nyp = y + j * step;
+      f.x = nxp;
+      f.y = nyp;
+      double e = 0;
+      if ((rc = pd_add(d, n)) || (rc = pd_update(d, n)) || (rc = pd_iterate_frame(d, n)) ||
+          (rc = fetch_error(f.r, &e)))
+        return rc;
+      if (e < best_error) {
+        best_error = e;
+        best_x = nxp;
+        best_y = nyp;
+      }
+      HIPCHK(hipMemcpyAsync(d->object, obj_copy, ob, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(f.result, single_copy, fb, hipMemcpyDeviceToDevice, c->stream));
+      if ((rc = pd_set_result(d, n))) return rc;
+    }
+  f.x = best_x;
+  f.y = best_y;
+  if (best_x == x && best_y == y) step = step / 2.0;
+  int st = 0;
+  if ((rc = pd_check_position(d, n, step, min_x, max_x, min_y, max_y, tries + 1, gave_up, &st))) return rc;
+  if (!st) {
+    f.x = x;
+    f.y = y;
+    *status = 0;
+    return 0;
+  }
+  d->beta = 1.0;
+  d->weights_set = false;
+  *status = 1;
+  return 0;
+}
+
+// adjust_positions (PhaseDiverseCDI.c++:337-579), on a temporary global
+// object of ones; the real object is left as it was.
+template <typename T>
+int pd_adjust_T(nadia_pd* d, int type, bool forward, int min_x, int max_x, int min_y, int max_y, double step,
+                int* gave_up) {
+  const int nf = (int)d->frames.size();
+  DeviceCtx* c = dev_ctx(d->dev);
+  const int lnx = d->frames[0].r->nx, lny = d->frames[0].r->ny, sc = d->scale;
+  const int mx = lnx * sc, my = lny * sc;
+  const int parallel_c = d->parallel, nx_c = d->nx, ny_c = d->ny, xmin_c = d->x_min, ymin_c = d->y_min;
+  const double beta_c = d->beta;
+  void* const obj_c = d->object;
+  DevScratch scr(c);
+  void* temp_object;
+  int rc;
+  if ((rc = scr.get(&temp_object, (size_t)d->nx * d->ny * sizeof(T) * 2))) return rc;
+  auto restore = [&]() {
+    d->object = obj_c;
+    d->nx = nx_c;
+    d->ny = ny_c;
+    d->x_min = xmin_c;
+    d->y_min = ymin_c;
+    d->parallel = parallel_c;
+    d->beta = beta_c;
+    d->weights_set = false;
+  };
+  d->parallel = 0;
+  d->object = temp_object;
+  {
+    const PdArgs a = pd_args(d, -1);
+    k_pd_scale<T>(a, nullptr, 0, 0.0, c->stream);  // every pixel 1 + 0i
+  }
+  d->beta = 1.0;
+  d->weights_set = false;
+  const int first = forward ? 0 : nf - 1;
+  if ((rc = pd_iterate_frame(d, first)) || (rc = pd_get_result(d, first)) || (rc = pd_add(d, first))) {
+    restore();
+    return rc;
+  }
+  for (int n = forward ? 1 : nf - 2; n != (forward ? nf : -1); n += forward ? 1 : -1) {
+    PdFrame& f = d->frames[n];
+    const double bx = f.x, by = f.y;
+    if ((rc = pd_iterate_frame(d, n)) || (rc = pd_get_result(d, n))) break;
+    XcMaps m{d->object, f.result, nullptr, nullptr, d->nx, d->ny, lnx, lny, mx, my, sc, d->x_min, d->y_min, bx, by};
+    DevScratch it(c);
+    if (type == 0) {
+      void *tothers, *tsingle;
+      if ((rc = it.get(&tothers, (size_t)mx * my * sizeof(T))) || (rc = it.get(&tsingle, (size_t)lnx * lny * sizeof(T))))
+        break;
+      m.others = tothers;
+      m.tsingle = tsingle;
+      k_xc_maps<T>(m, c->stream);
+      const void *img = tsingle, *wgt = f.r->S;
+      if (sc != 1) {
+        void *ibig, *wbig;
+        if ((rc = it.get(&ibig, (size_t)mx * my * sizeof(T))) || (rc = it.get(&wbig, (size_t)mx * my * sizeof(T))))
+          break;
+        k_interp<T>(tsingle, lnx, lny, ibig, mx, my, c->stream);
+        k_interp<T>(f.r->S, lnx, lny, wbig, mx, my, c->stream);
+        img = ibig;
+        wgt = wbig;
+      }
+      LAUNCHCHK();
+      int new_x = 0, new_y = 0, found = 0;
+      if ((rc = align_T<T>(c, img, mx, my, tothers, mx, my, &new_x, &new_y, &found, min_x, max_x, min_y, max_y, wgt,
+                           tothers, 0.2)))
+        break;
+      f.x = bx + new_x / (double)sc;
+      f.y = by + new_y / (double)sc;
+    } else {
+      void* temp;
+      if ((rc = it.get(&temp, (size_t)mx * my * sizeof(T) * 2))) break;
+      m.others = temp;
+      k_pd_subgrid<T>(m, c->stream);
+      LAUNCHCHK();
+      d->object = temp;
+      d->nx = mx;
+      d->ny = my;
+      d->x_min = (int)(-f.x);
+      d->y_min = (int)(-f.y);
+      int st = 0;
+      rc = pd_check_position(d, n, step, min_x, max_x, min_y, max_y, 0, gave_up ? &gave_up[n] : nullptr, &st);
+      d->object = temp_object;
+      d->nx = nx_c;
+      d->ny = ny_c;
+      d->x_min = xmin_c;
+      d->y_min = ymin_c;
+      if (rc) break;
+    }
+    d->weights_set = false;
+    if ((rc = pd_add(d, n))) break;
+  }
+  restore();
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -2271,6 +2507,59 @@ int nadia_pd_iterate(nadia_pd* d, double* frame_errors) {
     for (int n = 0; n < nf; n++)
       if ((rc = fetch_error(d->frames[n].r, &frame_errors[n]))) return rc;
   return 0;
+}
+
+// adjust_positions (PhaseDiverseCDI.c++:337-579)
+int nadia_pd_adjust_positions(nadia_pd* d, int type, int forward, int x_min, int x_max, int y_min, int y_max,
+                              double step_size, int* gave_up) {
+  if (!d || !d->object || d->frames.empty()) return fail(NADIA_ESTATE, "no frames");
+  if (type != 0 && type != 1)
+    return fail(NADIA_EINVAL, "The alignment type given to PhaseDiverseCDI::adjust_positions is not a known type");
+  for (const PdFrame& f : d->frames)
+    if (f.r->nx != d->frames[0].r->nx || f.r->ny != d->frames[0].r->ny)
+      return fail(NADIA_ESIZE, "adjust_positions needs frames of one size");
+  if (gave_up)
+    for (size_t n = 0; n < d->frames.size(); n++) gave_up[n] = 0;
+  int rc;
+  DISPATCH(d->prec, rc = pd_adjust_T<T>(d, type, forward != 0, x_min, x_max, y_min, y_max, step_size, gave_up));
+  return rc;
+}
+
+int nadia_pd_get_position(const nadia_pd* d, int n, double* x, double* y) {
+  if (!d || !x || !y) return fail(NADIA_EINVAL, "null argument");
+  if (n < 0 || n >= (int)d->frames.size()) return fail(NADIA_EINVAL, "no frame %d", n);
+  *x = d->frames[n].x;
+  *y = d->frames[n].y;
+  return 0;
+}
+
+// align (utils.c:1151-1343) on host arrays (x-major, element type of
+// `precision`); w1 / w2 may be null
+int nadia_align(int precision, const void* img1, int nx1, int ny1, const void* img2, int nx2, int ny2, int min_x,
+                int max_x, int min_y, int max_y, const void* w1, const void* w2, double overlap_fraction,
+                int* offset_x, int* offset_y, int* found) {
+  if (!img1 || !img2 || !offset_x || !offset_y || !found || nx1 <= 0 || ny1 <= 0 || nx2 <= 0 || ny2 <= 0)
+    return fail(NADIA_EINVAL, "bad argument");
+  if (precision != NADIA_FP32 && precision != NADIA_FP64) return fail(NADIA_EINVAL, "unknown precision");
+  DeviceCtx* c = dev_ctx(current_device());
+  const size_t rs = precision == NADIA_FP64 ? 8 : 4;
+  DevScratch sc(c);
+  void *a, *b, *wa = nullptr, *wb = nullptr;
+  int rc;
+  if ((rc = sc.get(&a, (size_t)nx1 * ny1 * rs)) || (rc = sc.get(&b, (size_t)nx2 * ny2 * rs))) return rc;
+  HIPCHK(hipMemcpyAsync(a, img1, (size_t)nx1 * ny1 * rs, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(b, img2, (size_t)nx2 * ny2 * rs, hipMemcpyHostToDevice, c->stream));
+  if (w1) {
+    if ((rc = sc.get(&wa, (size_t)nx1 * ny1 * rs))) return rc;
+    HIPCHK(hipMemcpyAsync(wa, w1, (size_t)nx1 * ny1 * rs, hipMemcpyHostToDevice, c->stream));
+  }
+  if (w2) {
+    if ((rc = sc.get(&wb, (size_t)nx2 * ny2 * rs))) return rc;
+    HIPCHK(hipMemcpyAsync(wb, w2, (size_t)nx2 * ny2 * rs, hipMemcpyHostToDevice, c->stream));
+  }
+  DISPATCH(precision, rc = align_T<T>(c, a, nx1, ny1, b, nx2, ny2, offset_x, offset_y, found, min_x, max_x, min_y,
+                                      max_y, wa, wb, overlap_fraction));
+  return rc;
 }
 
 int nadia_pd_get_size(const nadia_pd* d, int* nx, int* ny) {

@@ -73,6 +73,22 @@ void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t s
   hipLaunchKernelGGL(final_jobs_kernel, dim3(njobs), dim3(256), 0, st, jobs, n, iter);
 }
 
+// best of the per-block XcBest partials: largest value, smallest index on ties
+__global__ void xc_final_kernel(const XcBest* partials, int n, XcBest* out) {
+  if (threadIdx.x != 0) return;
+  XcBest b{0.0, -1};
+  for (int k = 0; k < n; ++k) {
+    const XcBest c = partials[k];
+    if (c.index < 0) continue;
+    if (b.index < 0 || c.value > b.value || (c.value == b.value && c.index < b.index)) b = c;
+  }
+  out[0] = b;
+}
+
+void k_xc_final(const XcBest* partials, int n, XcBest* out, hipStream_t st) {
+  hipLaunchKernelGGL(xc_final_kernel, dim3(1), dim3(64), 0, st, partials, n, out);
+}
+
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st) {
   hipLaunchKernelGGL(final_sum2_kernel, dim3(1), dim3(256), 0, st, partials, n, out_ratio, out_pair);
 }

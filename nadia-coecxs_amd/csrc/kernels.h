@@ -150,6 +150,44 @@ template <typename T> void k_pd_add(const PdArgs& a, hipStream_t st);
 template <typename T> void k_pd_update(const PdArgs& a, hipStream_t st);
 template <typename T> void k_pd_scale(const PdArgs& a, const PdFrameDev* frames, int nframes, double factor,
                                       hipStream_t st);
+// PhaseDiverseCDI::adjust_positions (NADIA/src/PhaseDiverseCDI.c++:337-579):
+// the 1 - |T| images of the global sub-grid around a frame and of the frame
+// (cross-correlation), the object sub-grid (minimum error), bilinear
+// up-sampling (NADIA/src/utils.c:1346-1435) and the weighted
+// cross-correlation of align (NADIA/src/utils.c:1151-1343)
+struct XcMaps {
+  const void* object;  // nx x ny complex
+  const void* single;  // the frame's result, lnx x lny complex
+  void* others;        // mx x my real (or complex for the sub-grid copy)
+  void* tsingle;       // lnx x lny real
+  int nx, ny, lnx, lny, mx, my, scale, x_min, y_min;
+  double bx, by;
+};
+template <typename T> void k_xc_maps(const XcMaps& a, hipStream_t st);
+template <typename T> void k_pd_subgrid(const XcMaps& a, hipStream_t st);
+template <typename T> void k_interp(const void* src, int nx, int ny, void* dst, int bnx, int bny, hipStream_t st);
+struct XcPlace {
+  const void *img1, *w1, *img2, *w2;  // real images and optional weights
+  int nx1, ny1, nx2, ny2;
+  void *i1, *i2, *c1, *c2;            // NX x NY complex outputs (zeroed by the caller)
+  int nx, ny;
+  unsigned* count;                    // weight sum (zeroed by the caller)
+};
+template <typename T> void k_xc_place(const XcPlace& a, hipStream_t st);
+template <typename T> void k_xc_mulconj(void* a, const void* b, long long n, hipStream_t st);
+struct XcBest {
+  double value;
+  long long index;  // -1: none
+};
+struct XcScan {
+  const void *ci, *cw;  // correlations of the images and of the weights (complex)
+  int nx, ny;
+  double overlap_factor, overlap_fraction;
+  int min_x, max_x, min_y, max_y;
+};
+// best ratio (first in index order on ties) over kEwBlocks partials, then one
+template <typename T> void k_xc_scan(const XcScan& a, XcBest* partials, hipStream_t st);
+void k_xc_final(const XcBest* partials, int n, XcBest* out, hipStream_t st);
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st);
 // per-job final error reduction of the white-field row pass (grid = njobs)
 void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st);

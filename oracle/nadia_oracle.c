@@ -930,7 +930,7 @@ int oracle_real_size(void) { return (int)sizeof(real); }
 
 /* ------------------------------------------------------------------------ */
 /* PhaseDiverseCDI: NADIA/src/PhaseDiverseCDI.c++ with                       */
-/* NADIA/include/PhaseDiverseCDI.h (position refinement not modelled)        */
+/* NADIA/include/PhaseDiverseCDI.h (position refinement: see below)         */
 /* ------------------------------------------------------------------------ */
 
 #define OPD_MAX 64
