@@ -60,6 +60,11 @@ int fail(int code, const char* fmt, ...) {
 struct DeviceCtx {
   hipStream_t stream = nullptr;
   std::map<std::pair<int, int>, void*> twiddles;  // (n, precision) -> table
+  // side streams (default priority) and events that fork independent groups
+  // of a batch off `stream` while a graph is captured
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
 };
 
 std::mutex g_mu;
@@ -2668,10 +2673,22 @@ struct BatchLaunch {
   }
 };
 
-// Capture `steps` iterations of every launch group into a graph, in launch
-// order on the device stream.  (Groups are not put on separate streams: run
-// concurrently from several streams, launches of same-size FP64 groups gave
-// results that differed from serial execution by 1e-9 .. 1e-6.)
+int ensure_side(DeviceCtx* c, int n) {
+  if (!c->fork) HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+  while ((int)c->side.size() < n) {
+    hipStream_t st;
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    c->side.push_back(st);
+    hipEvent_t e;
+    HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    c->join.push_back(e);
+  }
+  return 0;
+}
+
+// Capture `steps` iterations of every launch group into a graph: each group
+// is an independent chain of passes (groups touch disjoint jobs), forked off
+// the device stream so the graph holds them as parallel branches.
 int capture_batch_graph(nadia_batch* b, DeviceCtx* c, const std::vector<BatchLaunch>& launches, int steps) {
   std::vector<long long> sig{steps, (long long)(intptr_t)b->djobs};
   for (const BatchLaunch& L : launches) {
@@ -2685,14 +2702,27 @@ int capture_batch_graph(nadia_batch* b, DeviceCtx* c, const std::vector<BatchLau
     (void)hipGraphExecDestroy(b->gexec);
     b->gexec = nullptr;
   }
+  const int nl = (int)launches.size();
+  int rc;
+  if ((rc = ensure_side(c, nl - 1))) return rc;
   hipStream_t st = c->stream;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  int rc = 0;
-  for (int k = 0; k < steps && !rc; ++k)
-    for (size_t i = 0; i < launches.size() && !rc; ++i) {
-      const BatchLaunch& L = launches[i];
-      DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, (int)L.rem.size(), k, L.cv, L.rv, nullptr, nullptr, st));
+  rc = 0;
+  if (nl > 1 && hipEventRecord(c->fork, st) != hipSuccess) rc = fail(NADIA_EHIP, "event record in capture failed");
+  for (int i = 0; i < nl && !rc; ++i) {
+    const BatchLaunch& L = launches[i];
+    hipStream_t s2 = i == 0 ? st : c->side[i - 1];
+    if (i > 0 && hipStreamWaitEvent(s2, c->fork, 0) != hipSuccess) {
+      rc = fail(NADIA_EHIP, "stream wait in capture failed");
+      break;
     }
+    for (int k = 0; k < steps && !rc; ++k)
+      DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, (int)L.rem.size(), k, L.cv, L.rv, nullptr, nullptr, s2));
+    if (!rc && i > 0 && hipEventRecord(c->join[i - 1], s2) != hipSuccess)
+      rc = fail(NADIA_EHIP, "event record in capture failed");
+  }
+  for (int i = 1; i < nl && !rc; ++i)
+    if (hipStreamWaitEvent(st, c->join[i - 1], 0) != hipSuccess) rc = fail(NADIA_EHIP, "stream join in capture failed");
   hipGraph_t graph = nullptr;
   hipError_t e = hipStreamEndCapture(st, &graph);
   if (rc) {

@@ -37,12 +37,18 @@ def test_iterate_each_matches_alone(nd, synth):
             assert g.get_error() == e0, j
 
 
-def test_graph_sweeps_match_eager(nd, synth):
-    spec = [(256, 31, nd.RAAR), (384, 32, nd.RAAR), (512, 33, nd.HIO), (256, 34, nd.ER), (768, 35, nd.RAAR)]
-    eager = [planar_job(nd, synth, n, s, a, double=False) for n, s, a in spec]
+@pytest.mark.parametrize("double", [False, True], ids=["fp32", "fp64"])
+def test_graph_sweeps_match_eager(nd, synth, double):
+    """Graph replays run the launch groups as parallel branches; every job
+    ends bit-identical to eager serial launches.  The 256^2 jobs form two
+    groups of one size (one and two work fields), the case that once showed
+    FP64 differences from a kernel store hazard under concurrency."""
+    spec = [(256, 31, nd.RAAR), (384, 32, nd.RAAR), (512, 33, nd.HIO), (256, 34, nd.ER), (768, 35, nd.RAAR),
+            (256, 36, nd.DM)]
+    eager = [planar_job(nd, synth, n, s, a, double=double) for n, s, a in spec]
     be = nd.Batch([g for _, g in eager])
     e_eager = np.vstack([be.iterate(25), be.iterate(25), be.iterate(7)])
-    graph = [planar_job(nd, synth, n, s, a, double=False) for n, s, a in spec]
+    graph = [planar_job(nd, synth, n, s, a, double=double) for n, s, a in spec]
     bg = nd.Batch([g for _, g in graph])
     bg.set_graph(10)
     # eager host work between graph replays (downloads, another batch)
