@@ -2843,16 +2843,35 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       eager_from = reps * G;
       last_slot = eager_from < chunk ? chunk - eager_from - 1 : G - 1;
     }
-    for (int k = eager_from; k < chunk; ++k) {
-      const int slot = k - eager_from;
-      for (const BatchLaunch& L : launches) {
+    // eager iterations: each launch group runs its chain of passes on its own
+    // stream (groups share no memory), forked off the device stream and
+    // joined back before the errors are read; unfused jobs stay on the
+    // device stream
+    const int nl = (int)launches.size();
+    const bool fork = nl > 1 && eager_from < chunk;
+    if (fork) {
+      if ((rc = ensure_side(c, nl - 1))) return rc;
+      HIPCHK(hipEventRecord(c->fork, c->stream));
+      for (int i = 1; i < nl; ++i) HIPCHK(hipStreamWaitEvent(c->side[i - 1], c->fork, 0));
+    }
+    for (int i = 0; i < nl; ++i) {
+      const BatchLaunch& L = launches[i];
+      hipStream_t s2 = (fork && i > 0) ? c->side[i - 1] : c->stream;
+      for (int k = eager_from; k < chunk; ++k) {
         const int a = L.active(k);
-        if (a == 0) continue;
-        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, slot, L.cv, L.rv, nullptr, nullptr));
+        if (a == 0) break;  // jobs are sorted by remaining count: none later either
+        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, k - eager_from, L.cv, L.rv, nullptr, nullptr, s2));
         if (rc) return rc;
       }
+    }
+    for (int k = eager_from; k < chunk; ++k)
       for (int j : plain)
-        if (rem[j] > k && (rc = iterate_plain(b->jobs[j], b->jobs[j]->errs + slot))) return rc;
+        if (rem[j] > k && (rc = iterate_plain(b->jobs[j], b->jobs[j]->errs + (k - eager_from)))) return rc;
+    if (fork) {
+      for (int i = 1; i < nl; ++i) {
+        HIPCHK(hipEventRecord(c->join[i - 1], c->side[i - 1]));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->join[i - 1], 0));
+      }
     }
     if (eager_from < chunk && (rc = emit(done + eager_from, chunk - eager_from))) return rc;
     for (int j = 0; j < nj; ++j) {
