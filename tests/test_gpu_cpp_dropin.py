@@ -111,3 +111,46 @@ def test_cpp_phase_diverse_with_constraint(oracle, synth, tmp_path):
     f0 = files.read_cplx(tmp_path / "frame0.cplx", n, n)
     assert np.linalg.norm(f0 - xs[0]) / np.linalg.norm(xs[0]) <= 1e-10
     assert abs(c_gpu - tc.c_mean(k)) <= 1e-10 * abs(tc.c_mean(k))
+
+
+@pytest.mark.parametrize("kind,forward", [(0, 1), (1, 0)], ids=["cross_correlation", "minimum_error_backward"])
+def test_cpp_adjust_positions(oracle, tmp_path, kind, forward):
+    """PhaseDiverseCDI::adjust_positions through the C++ drop-in (align_example)
+    against the oracle and the known frame offsets."""
+    import importlib
+    files = importlib.import_module("nadia-coecxs_amd.files")
+    synth = importlib.import_module("nadia-coecxs_amd.synth")
+    d = synth.ptycho_frames()
+    truth = d["positions"]
+    ref = 0 if forward else len(truth) - 1
+    starts = [(x + (0 if k == ref else 2.0 * (-1) ** k), y + (0 if k == ref else -3.0 + k))
+              for k, (x, y) in enumerate(truth)]
+    m = d["frames"][0].shape[0]
+    for k, (f, inten, sup) in enumerate(zip(d["frames"], d["intensities"], d["supports"])):
+        files.write_cplx(tmp_path / ("x0_%d.cplx" % k), f)
+        files.write_dbin(tmp_path / ("intensity_%d.dbin" % k), inten)
+        files.write_dbin(tmp_path / ("support_%d.dbin" % k), sup)
+    args = [str(v) for xy in starts for v in xy]
+    out = subprocess.run([os.path.join(CPP, "align_exampled"), str(tmp_path), str(m), str(kind), str(forward)] + args,
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    got = {}
+    for line in out.stdout.splitlines():
+        if line.startswith("position "):
+            _, k, x, y = line.split()
+            got[int(k)] = (float(x), float(y))
+    assert out.stdout.count("moving prob") == len(truth) - 1
+    objs = []
+    pd = oracle.OraclePhaseDiverse(beta=1.0)
+    for f, inten, sup, (x0, y0) in zip(d["frames"], d["intensities"], d["supports"], starts):
+        x = f.astype(np.complex128).copy()
+        o = oracle.OracleCDI(x, oracle.PLANAR)
+        o.set_support(sup)
+        o.set_intensity(inten)
+        o.set_algorithm(oracle.ER)
+        objs.append((x, o))
+        pd.add_new_position(o, x0, y0)
+    pd.initialise_estimate()
+    pd.adjust_positions(kind, bool(forward))
+    for k in range(len(truth)):
+        assert got[k] == pd.get_position(k) == truth[k]
