@@ -226,6 +226,41 @@ def leg_fresnel(nd, synth, n=1024, wf_iters=50, er_iters=300):
     return out
 
 
+def leg_reference_example(nd, synth):
+    """The reference's own example: the measured 1024^2 pattern of
+    NADIA/examples/image_files/planar_data.tif through the schedule of
+    NADIA/examples/PlanarCDI_example.c (2 cycles of 50 ER, 100 HIO at
+    relaxation 0.5, 50 ER, shrinkwrap every 50, 4 best estimates kept --
+    which needs each error on the host, so one call per iteration as the
+    example does), FP32 and FP64: wall time of the whole schedule and the
+    final errors."""
+    d = synth.reference_planar_example(os.path.join(ROOT, "tests", "golden", "planar_data.npz"))
+    n = d["intensity"].shape[0]
+    out = {"n": n, "input": "NADIA/examples/image_files/planar_data.tif",
+           "schedule": "PlanarCDI_example.c: 2 x (50 ER, 100 HIO beta 0.5, 50 ER), shrinkwrap every 50, n_best 4"}
+    fin = {}
+    for tag, double in (("fp32", False), ("fp64", True)):
+        est = nd.Complex2D(n, n, double=double)
+        p = nd.PlanarCDI(est, n_best=4)
+        p.set_support(d["support"])
+        p.set_intensity(d["intensity"])
+        p.initialise_estimate(0)
+        algs = {"ER": nd.ER, "HIO": nd.HIO}
+        t0 = time.perf_counter()
+        errs = synth.run_reference_example(
+            p, alg=lambda a: p.set_algorithm(algs[a]), iterate=lambda: float(p.iterate(1)[0]),
+            shrinkwrap=p.apply_shrinkwrap, set_beta=p.set_relaxation_parameter)
+        nd.synchronize()
+        dt = time.perf_counter() - t0
+        best, best_err = p.get_best_result(0)
+        out[tag] = {"iterations": len(errs), "seconds": round(dt, 3), "iter_s": round(len(errs) / dt, 1),
+                    "final_error": errs[-1], "best_error": best_err}
+        fin[tag] = est.get()
+        del p, est
+    out["fp32_vs_fp64_rel_l2"] = rel_l2(fin["fp32"], fin["fp64"])
+    return out
+
+
 def leg_phase_diverse(nd, synth, n=1024, frames=8, big_iters=10):
     """PhaseDiverseCDI: `frames` Fresnel frames of n^2 (the 1024^2 Fresnel
     workload, shifted by 16 px per frame along x) merged in parallel mode,
@@ -304,6 +339,8 @@ def main():
     ap.add_argument("--no-fresnel", action="store_true", help="skip the 1024^2 Fresnel workload")
     ap.add_argument("--no-served", action="store_true", help="skip the request stream")
     ap.add_argument("--no-phase-diverse", action="store_true", help="skip the PhaseDiverseCDI leg")
+    ap.add_argument("--no-reference-example", action="store_true",
+                    help="skip the reference's PlanarCDI example (measured 1024^2 pattern)")
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
@@ -413,6 +450,10 @@ def main():
     if not args.no_fresnel:
         extra["fresnel1024"] = leg_fresnel(nd, synth)
         print("bench: fresnel %s" % json.dumps(extra["fresnel1024"]), file=sys.stderr, flush=True)
+    if not args.no_reference_example:
+        extra["reference_example_1024"] = leg_reference_example(nd, synth)
+        print("bench: reference example %s" % json.dumps(extra["reference_example_1024"]), file=sys.stderr,
+              flush=True)
     if not args.no_phase_diverse:
         extra["phase_diverse"] = leg_phase_diverse(nd, synth)
         print("bench: phase diverse %s" % json.dumps(extra["phase_diverse"]), file=sys.stderr, flush=True)

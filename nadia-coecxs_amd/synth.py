@@ -270,3 +270,54 @@ def ptycho_frames(m=64, crops=((0, 0), (14, 9), (27, 20)), seed=7):
         ints.append(far_field(f))
     return dict(object=obj, frames=frames, intensities=ints, supports=[np.ones((m, m)) for _ in crops],
                 positions=[(-float(X), -float(Y)) for X, Y in crops])
+
+
+def reference_planar_example(npz_path, threshold=0.04):
+    """The reference's example input for PlanarCDI (the 1024 x 1024 uint16
+    diffraction pattern NADIA/examples/image_files/planar_data.tif, kept as
+    tests/golden/planar_data.npz), oriented as NADIA's TIFF reader loads it
+    (x = column, y = row: intensity[x, y] = pixel at row y, column x).  The
+    example's support file is not part of the reference's distribution; the
+    support here is the set where the intensity autocorrelation
+    |IDFT(I)| (centred) exceeds `threshold` of its maximum -- about twice the
+    object's extent, which the example's shrinkwrap steps then tighten.
+    Returns dict(intensity, support)."""
+    img = np.load(npz_path)["rows_cols"].astype(np.float64)
+    inten = np.ascontiguousarray(img.T)
+    a = np.abs(np.fft.fftshift(np.fft.ifft2(np.fft.ifftshift(inten))))
+    support = (a > threshold * a.max()).astype(np.float64)
+    return dict(intensity=inten, support=support)
+
+
+# NADIA/examples/PlanarCDI_example.c:40-222: per cycle 50 ER (shrinkwrap
+# sigma 1.5 every 50), relaxation 0.5 then 100 HIO and 50 ER (shrinkwrap
+# sigma 2.0 every 50), threshold 0.1, two cycles, 4 best estimates kept
+REFERENCE_EXAMPLE_SCHEDULE = dict(cycles=2, er1=50, hio=100, er2=50, shrinkwrap_every=50, sigma_er1=1.5,
+                                  sigma_rest=2.0, threshold=0.1, beta=0.5, n_best=4, seed=0)
+
+
+def run_reference_example(obj, alg, iterate, shrinkwrap, set_beta, cycles=None, sched=REFERENCE_EXAMPLE_SCHEDULE):
+    """Drives `obj` (anything with the callables given) through the example's
+    loop: iterate(), shrinkwrap(sigma, threshold), set_beta(b), alg(name).
+    Returns the errors after each iteration."""
+    s = sched
+    errs = []
+    cyc = s["cycles"] if cycles is None else cycles
+    for _ in range(cyc):
+        alg("ER")
+        for i in range(s["er1"]):
+            errs.append(iterate())
+            if i % s["shrinkwrap_every"] == s["shrinkwrap_every"] - 1:
+                shrinkwrap(s["sigma_er1"], s["threshold"])
+        set_beta(s["beta"])
+        alg("HIO")
+        for i in range(s["er1"], s["er1"] + s["hio"]):
+            errs.append(iterate())
+            if i % s["shrinkwrap_every"] == s["shrinkwrap_every"] - 1:
+                shrinkwrap(s["sigma_rest"], s["threshold"])
+        alg("ER")
+        for i in range(s["er1"] + s["hio"], s["er1"] + s["hio"] + s["er2"]):
+            errs.append(iterate())
+            if i % s["shrinkwrap_every"] == s["shrinkwrap_every"] - 1:
+                shrinkwrap(s["sigma_rest"], s["threshold"])
+    return errs
