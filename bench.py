@@ -230,10 +230,10 @@ def leg_reference_example(nd, synth):
     """The reference's own example: the measured 1024^2 pattern of
     NADIA/examples/image_files/planar_data.tif through the schedule of
     NADIA/examples/PlanarCDI_example.c (2 cycles of 50 ER, 100 HIO at
-    relaxation 0.5, 50 ER, shrinkwrap every 50, 4 best estimates kept --
-    which needs each error on the host, so one call per iteration as the
-    example does), FP32 and FP64: wall time of the whole schedule and the
-    final errors."""
+    relaxation 0.5, 50 ER, shrinkwrap every 50, 4 best estimates kept on
+    the device), one iterate() call and one get_error() per iteration as the
+    example's loop does (it prints every error), FP32 and FP64: wall time of
+    the whole schedule and the final errors."""
     d = synth.reference_planar_example(os.path.join(ROOT, "tests", "golden", "planar_data.npz"))
     n = d["intensity"].shape[0]
     out = {"n": n, "input": "NADIA/examples/image_files/planar_data.tif",

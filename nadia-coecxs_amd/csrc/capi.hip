@@ -180,9 +180,13 @@ struct nadia_recon {
   // white field
   double z12, z23, wf_sx, wf_sy, wf_k, wf_norm;
   bool wf_ready;
-  // n best
+  // n best: snapshot buffers (fixed), and on the device the kept errors by
+  // rank, the rank -> buffer map, the buffer pointers and the last copy target
   std::vector<void*> best;
-  std::vector<double> best_err;
+  double* dbest_err;
+  int* dbest_perm;
+  void** dbest_bufs;
+  int* dbest_target;
   // tile variants of the fused passes (-1: not chosen yet)
   int col_v, row_v;
   bool tile_fixed;  // set by the caller, not the autotuner
@@ -785,15 +789,21 @@ void fused_done(nadia_recon* r, int n) {
   r->h_err_valid = false;
 }
 
+bool wants_n_best(const nadia_recon* r);
+int update_n_best(nadia_recon* r, const double* err, hipStream_t st);
+
 template <typename T>
 int iterate_fused(nadia_recon* r, int n) {
   int rc;
   if ((rc = fused_ready<T>(r))) return rc;
   std::vector<Ev> evs;
   const GroupKey g = group_key(r);
-  for (int k = 0; k < n; ++k)
+  const bool nb = wants_n_best(r);
+  for (int k = 0; k < n; ++k) {
     if ((rc = launch_fused<T>(r->ctx, g, r->djob, 1, k, r->col_v, r->row_v, r, r->timing ? &evs : nullptr)))
       return rc;
+    if (nb && (rc = update_n_best(r, r->errs + k, S_(r)))) return rc;
+  }
   fused_done(r, n);
   return collect_events(r, evs);
 }
@@ -816,24 +826,22 @@ int fetch_error(nadia_recon* r, double* out) {
   return 0;
 }
 
-// BaseCDI::update_n_best (NADIA/src/BaseCDI.c++:390-413)
-int update_n_best(nadia_recon* r) {
-  if (r->n_best <= 0) return 0;
-  double err = 0;
-  int rc;
-  if ((rc = fetch_error(r, &err))) return rc;
-  int place = 0;
-  for (; place < r->n_best && err > r->best_err[place]; place++) {}
-  if (place < r->n_best) {
-    void* tp = r->best[r->n_best - 1];
-    HIPCHK(hipMemcpyAsync(tp, r->est->d, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
-    for (int i = r->n_best - 1; i > place; i--) {
-      r->best_err[i] = r->best_err[i - 1];
-      r->best[i] = r->best[i - 1];
-    }
-    r->best_err[place] = err;
-    r->best[place] = tp;
-  }
+// BaseCDI::update_n_best (NADIA/src/BaseCDI.c++:390-413), queued on the
+// device after an iteration: `err` is the device slot holding that
+// iteration's get_error() (its ring slot; for a projection-free algorithm the
+// unchanged current error).  No host round trip.
+bool wants_n_best(const nadia_recon* r) { return r->n_best > 0 && r->kind != NADIA_FRESNEL_WF; }
+
+const double* current_error_slot(const nadia_recon* r, const double* ring_slot) {
+  if (has_projection(r)) return ring_slot;
+  return r->last_err ? r->last_err : r->scal + 10;  // never projected: 0, as get_error() reports
+}
+
+int update_n_best(nadia_recon* r, const double* err, hipStream_t st) {
+  if (!wants_n_best(r)) return 0;
+  k_nbest(err, r->dbest_err, r->dbest_perm, r->n_best, r->dbest_target, r->est->d, r->dbest_bufs,
+          (long long)(r->npix * r->esz), st);
+  LAUNCHCHK();
   return 0;
 }
 
@@ -851,6 +859,7 @@ int support_from_rtmp(nadia_recon* r, int soften) {
     std::vector<float> gf(g.begin(), g.end());
     if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g.data(), 36 * 8, hipMemcpyHostToDevice, S_(r)));
     else HIPCHK(hipMemcpyAsync(r->gwin, gf.data(), 36 * 4, hipMemcpyHostToDevice, S_(r)));
+    HIPCHK(hipStreamSynchronize(S_(r)));  // g / gf end with this scope
     ConvArgs ca{};
     ca.src = r->S;
     ca.src_complex = 0;
@@ -1082,6 +1091,7 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   }
   (void)hipMemsetAsync(r->S, 0, rb, S_(r));
   (void)hipMemsetAsync(r->Q, 0, rb, S_(r));
+  (void)hipMemsetAsync(r->scal, 0, sizeof(double) * 16, S_(r));
   for (int i = 0; i < n_best; i++) {
     void* p = nullptr;
     if (hipMalloc(&p, cb) != hipSuccess) {
@@ -1090,7 +1100,23 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
     }
     (void)hipMemsetAsync(p, 0, cb, S_(r));
     r->best.push_back(p);
-    r->best_err.push_back(1.0);
+  }
+  if (n_best > 0) {
+    if ((rc = alloc((void**)&r->dbest_err, sizeof(double) * n_best)) ||
+        (rc = alloc((void**)&r->dbest_perm, sizeof(int) * n_best)) ||
+        (rc = alloc((void**)&r->dbest_bufs, sizeof(void*) * n_best)) || (rc = alloc((void**)&r->dbest_target, sizeof(int)))) {
+      delete r;
+      return rc;
+    }
+    // BaseCDI constructor: every best error starts at 1 (src/BaseCDI.c++:50-56)
+    std::vector<double> e(n_best, 1.0);
+    std::vector<int> perm(n_best);
+    for (int i = 0; i < n_best; i++) perm[i] = i;
+    HIPCHK(hipMemcpyAsync(r->dbest_err, e.data(), sizeof(double) * n_best, hipMemcpyHostToDevice, S_(r)));
+    HIPCHK(hipMemcpyAsync(r->dbest_perm, perm.data(), sizeof(int) * n_best, hipMemcpyHostToDevice, S_(r)));
+    HIPCHK(hipMemcpyAsync(r->dbest_bufs, r->best.data(), sizeof(void*) * n_best, hipMemcpyHostToDevice, S_(r)));
+    // the host arrays above end with this block: the copies must be done
+    HIPCHK(hipStreamSynchronize(S_(r)));
   }
   HIPCHK(hipStreamSynchronize(S_(r)));
   // BaseCDI::BaseCDI: algorithm = CUSTOM; set_algorithm(ER) (src/BaseCDI.c++:63-64)
@@ -1146,7 +1172,8 @@ int nadia_recon_destroy(nadia_recon* r) {
   if (!r) return 0;
   (void)hipStreamSynchronize(S_(r));
   void* ptrs[] = {r->S, r->Q, r->beam, r->W, r->t[0], r->t[1], r->t[2], r->t[3], r->conv, r->rtmp,
-                  r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs, r->ttmp};
+                  r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs, r->ttmp,
+                  r->dbest_err, r->dbest_perm, r->dbest_bufs, r->dbest_target};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (void* p : r->best) (void)hipFree(p);
@@ -1272,22 +1299,21 @@ int nadia_recon_iterate(nadia_recon* r, int n, double* errors) {
   if (!r->has_intensity) return fail(NADIA_ESTATE, "intensity not set");
   int rc;
   if ((rc = need_setup(r))) return rc;
-  const bool nb = r->n_best > 0 && r->kind != NADIA_FRESNEL_WF;
+  const bool nb = wants_n_best(r);
   int done = 0;
   while (done < n) {
     int chunk = n - done;
     if (chunk > r->ring_cap) chunk = r->ring_cap;
-    if (nb) chunk = 1;  // the n-best bookkeeping needs each error on the host
     if (fused_ok(r)) {
       DISPATCH(r->prec, rc = iterate_fused<T>(r, chunk));
       if (rc) return rc;
     } else {
       for (int k = 0; k < chunk; ++k) {
         if ((rc = iterate_plain(r, r->errs + k))) return rc;
+        if (nb && (rc = update_n_best(r, current_error_slot(r, r->errs + k), S_(r)))) return rc;
       }
       r->est->version++;
     }
-    if (nb && (rc = update_n_best(r))) return rc;
     if (errors) {
       if (has_projection(r)) {
         HIPCHK(hipMemcpyAsync(errors + done, r->errs, chunk * sizeof(double), hipMemcpyDeviceToHost, S_(r)));
@@ -1328,6 +1354,9 @@ int nadia_recon_apply_shrinkwrap(nadia_recon* r, double gauss_width, double thre
   for (int i = 0; i < 25; i++) gf[i] = (float)g[i];
   if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g, sizeof(g), hipMemcpyHostToDevice, S_(r)));
   else HIPCHK(hipMemcpyAsync(r->gwin, gf, sizeof(gf), hipMemcpyHostToDevice, S_(r)));
+  // the window lives on this stack frame: the copy must be done before the
+  // function returns (a pageable host copy may be read after the call)
+  HIPCHK(hipStreamSynchronize(S_(r)));
   ConvArgs ca{};
   ca.src = r->est->d;
   ca.src_complex = 1;
@@ -1483,17 +1512,26 @@ int nadia_recon_get_best_result(nadia_recon* r, int index, double* error, void* 
     fprintf(stdout, "index out of bounds in BaseCDI::get_best_result\n");
     return fail(NADIA_EINVAL, "index out of bounds in get_best_result");
   }
-  if (error) *error = r->best_err[index];
+  double e = 0;
+  int buf = 0;
+  HIPCHK(hipMemcpyAsync(&e, r->dbest_err + index, sizeof(double), hipMemcpyDeviceToHost, S_(r)));
+  HIPCHK(hipMemcpyAsync(&buf, r->dbest_perm + index, sizeof(int), hipMemcpyDeviceToHost, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));
+  if (error) *error = e;
   if (host) {
-    HIPCHK(hipMemcpyAsync(host, r->best[index], r->npix * r->esz, hipMemcpyDeviceToHost, S_(r)));
+    HIPCHK(hipMemcpyAsync(host, r->best[buf], r->npix * r->esz, hipMemcpyDeviceToHost, S_(r)));
     HIPCHK(hipStreamSynchronize(S_(r)));
   }
   return 0;
 }
 
+// reset_best (NADIA/include/BaseCDI.h:476-479): every kept error back to 1
 int nadia_recon_reset_best(nadia_recon* r) {
   if (!r) return fail(NADIA_EINVAL, "null recon");
-  for (int i = 0; i < r->n_best; i++) r->best_err[i] = 1;
+  if (r->n_best <= 0) return 0;
+  std::vector<double> e(r->n_best, 1.0);
+  HIPCHK(hipMemcpyAsync(r->dbest_err, e.data(), sizeof(double) * r->n_best, hipMemcpyHostToDevice, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));
   return 0;
 }
 
@@ -1674,6 +1712,7 @@ int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size) {
   std::vector<float> gf(g.begin(), g.end());
   if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g.data(), 36 * 8, hipMemcpyHostToDevice, S_(r)));
   else HIPCHK(hipMemcpyAsync(r->gwin, gf.data(), 36 * 4, hipMemcpyHostToDevice, S_(r)));
+  HIPCHK(hipStreamSynchronize(S_(r)));  // g / gf end with this function
   ConvArgs ca{};
   ca.src = r->rtmp;
   ca.src_complex = 0;
@@ -2666,6 +2705,15 @@ struct BatchLaunch {
   JobDesc* d;
   int cv, rv;
   std::vector<int> rem;
+  std::vector<nadia_recon*> jobs;  // in descriptor order
+  // after iteration k (ring slot `slot`) of the first a jobs: their n-best
+  // bookkeeping, on the group's stream
+  int n_best_after(int a, int slot, hipStream_t st) const {
+    int rc;
+    for (int q = 0; q < a; ++q)
+      if (wants_n_best(jobs[q]) && (rc = update_n_best(jobs[q], jobs[q]->errs + slot, st))) return rc;
+    return 0;
+  }
   int active(int k) const {
     int a = 0;
     while (a < (int)rem.size() && rem[a] > k) a++;
@@ -2716,8 +2764,10 @@ int capture_batch_graph(nadia_batch* b, DeviceCtx* c, const std::vector<BatchLau
       rc = fail(NADIA_EHIP, "stream wait in capture failed");
       break;
     }
-    for (int k = 0; k < steps && !rc; ++k)
+    for (int k = 0; k < steps && !rc; ++k) {
       DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, (int)L.rem.size(), k, L.cv, L.rv, nullptr, nullptr, s2));
+      if (!rc) rc = L.n_best_after((int)L.rem.size(), k, s2);
+    }
     if (!rc && i > 0 && hipEventRecord(c->join[i - 1], s2) != hipSuccess)
       rc = fail(NADIA_EHIP, "event record in capture failed");
   }
@@ -2756,13 +2806,11 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
   DeviceCtx* c = dev_ctx(b->dev);
   if (!c) return fail(NADIA_EHIP, "stream creation failed");
   int rc;
-  bool per_iteration = false;
   for (int j = 0; j < nj; ++j) {
     nadia_recon* r = b->jobs[j];
     if (counts[j] == 0) continue;
     if (!r->has_intensity) return fail(NADIA_ESTATE, "a reconstruction in the batch has no intensity");
     if ((rc = need_setup(r))) return rc;
-    if (r->n_best > 0 && r->kind != NADIA_FRESNEL_WF) per_iteration = true;
   }
   if (b->cap < nj) {
     if (b->gexec) {  // the graph addresses the old descriptor array
@@ -2798,7 +2846,6 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
   while (done < nmax) {
     int chunk = nmax - done;
     if (chunk > 4096) chunk = 4096;
-    if (per_iteration) chunk = 1;
     std::vector<int> rem(nj);
     for (int j = 0; j < nj; ++j) rem[j] = std::max(0, std::min(chunk, counts[j] - done));
     // group the fused-path jobs; the rest run the unfused path one by one
@@ -2820,17 +2867,18 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
         if (rc) return rc;
       }
       nadia_recon* r0 = b->jobs[kv.second[0]];
-      BatchLaunch L{kv.first, b->djobs + host.size(), r0->col_v, r0->row_v, {}};
+      BatchLaunch L{kv.first, b->djobs + host.size(), r0->col_v, r0->row_v, {}, {}};
       for (int j : kv.second) {
         host.push_back(make_jobdesc(b->jobs[j], b->jobs[j]->est->d, b->jobs[j]->W, b->jobs[j]->errs));
         L.rem.push_back(rem[j]);
+        L.jobs.push_back(b->jobs[j]);
       }
       launches.push_back(std::move(L));
     }
     if (!host.empty() && (rc = upload_jobs(c->stream, b->djobs, host.data(), (int)host.size()))) return rc;
     // graph replays for the uniform, fused-only part; the rest eagerly
     const int G = b->graph_steps;
-    const bool graph = G > 0 && uniform && plain.empty() && !per_iteration && !launches.empty() && chunk >= G;
+    const bool graph = G > 0 && uniform && plain.empty() && !launches.empty() && chunk >= G;
     int eager_from = 0, last_slot = chunk - 1;
     if (graph) {
       if ((rc = capture_batch_graph(b, c, launches, G))) return rc;
@@ -2861,12 +2909,17 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
         const int a = L.active(k);
         if (a == 0) break;  // jobs are sorted by remaining count: none later either
         DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, k - eager_from, L.cv, L.rv, nullptr, nullptr, s2));
-        if (rc) return rc;
+        if (rc || (rc = L.n_best_after(a, k - eager_from, s2))) return rc;
       }
     }
     for (int k = eager_from; k < chunk; ++k)
-      for (int j : plain)
-        if (rem[j] > k && (rc = iterate_plain(b->jobs[j], b->jobs[j]->errs + (k - eager_from)))) return rc;
+      for (int j : plain) {
+        nadia_recon* r = b->jobs[j];
+        if (rem[j] <= k) continue;
+        if ((rc = iterate_plain(r, r->errs + (k - eager_from)))) return rc;
+        if (wants_n_best(r) && (rc = update_n_best(r, current_error_slot(r, r->errs + (k - eager_from)), c->stream)))
+          return rc;
+      }
     if (fork) {
       for (int i = 1; i < nl; ++i) {
         HIPCHK(hipEventRecord(c->join[i - 1], c->side[i - 1]));
@@ -2879,7 +2932,6 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       nadia_recon* r = b->jobs[j];
       if (fused_ok(r)) fused_done(r, graph ? last_slot + 1 : rem[j]);
       else r->est->version++;
-      if (per_iteration && r->n_best > 0 && r->kind != NADIA_FRESNEL_WF && (rc = update_n_best(r))) return rc;
     }
     done += chunk;
     if (done < nmax) HIPCHK(hipStreamSynchronize(c->stream));

@@ -188,6 +188,10 @@ struct XcScan {
 // best ratio (first in index order on ties) over kEwBlocks partials, then one
 template <typename T> void k_xc_scan(const XcScan& a, XcBest* partials, hipStream_t st);
 void k_xc_final(const XcBest* partials, int n, XcBest* out, hipStream_t st);
+// update_n_best on the device: rank the error at `err`, then copy `bytes` of
+// the estimate `src` into bufs[new rank's buffer] when it placed
+void k_nbest(const double* err, double* best_err, int* perm, int n, int* target, const void* src, void* const* bufs,
+             long long bytes, hipStream_t st);
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st);
 // per-job final error reduction of the white-field row pass (grid = njobs)
 void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st);

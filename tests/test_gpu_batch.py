@@ -7,12 +7,12 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def planar_job(nd, synth, n, seed, alg, double=True, beta=0.9, beam=False):
+def planar_job(nd, synth, n, seed, alg, double=True, beta=0.9, beam=False, n_best=0):
     d = synth.planar_block(n=n, block=3 * n // 8, support=7 * n // 16, seed_amp=seed, seed_poisson=seed,
                            seed_phase=seed + 1)
     cdt = np.complex128 if double else np.complex64
     est = nd.Complex2D(n, n, double=double, data=d["x0"].astype(cdt))
-    g = nd.PlanarCDI(est)
+    g = nd.PlanarCDI(est, n_best=n_best)
     g.set_support(d["support"])
     g.set_intensity(d["intensity"])
     if beam:
@@ -86,3 +86,24 @@ def test_batch_fresnel_and_wf(nd, synth):
     e = b.iterate(5)
     assert np.array_equal(e[:, 0], r1) and np.array_equal(e[:, 1], r2)
     assert np.array_equal(f1.get(), e1.get()) and np.array_equal(f2.get(), e2.get())
+
+
+@pytest.mark.parametrize("graph", [0, 4], ids=["eager", "graph"])
+def test_batch_n_best_matches_alone(nd, synth, graph):
+    """Best estimates kept on the device inside batched launches (eager and
+    graph replays) equal those of the same jobs iterated alone."""
+    spec = [(256, 61, nd.HIO), (256, 62, nd.ER), (128, 63, nd.RAAR)]
+    alone = []
+    for n, seed, alg in spec:
+        est, g = planar_job(nd, synth, n, seed, alg, n_best=3)
+        g.iterate(9)
+        alone.append([g.get_best_result(k) for k in range(3)])
+    jobs = [planar_job(nd, synth, n, seed, alg, n_best=3) for n, seed, alg in spec]
+    b = nd.Batch([g for _, g in jobs])
+    if graph:
+        b.set_graph(graph)
+    b.iterate(9)
+    for (est, g), want in zip(jobs, alone):
+        for k in range(3):
+            x, e = g.get_best_result(k)
+            assert np.array_equal(x, want[k][0]) and e == want[k][1], k

@@ -206,18 +206,40 @@ def test_initialise_estimate_planar(nd, oracle, synth):
     assert rel(est.get(), xo) <= 1e-15
 
 
-def test_n_best(nd, oracle, synth):
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+def test_n_best(nd, oracle, synth, path):
+    """update_n_best runs on the device after every iteration (no host round
+    trip); kept errors and snapshots follow the oracle through reset_best
+    and a projection-free custom algorithm (its unchanged error ranks)."""
     d = small_state(synth, n=64)
     est, g, xo, o = make_pair(nd, oracle, d, n_best=3)
+    g.set_path(path)
     for obj in (g, o):
         obj.set_algorithm(nd.HIO)
     g.iterate(6)
     o.iterate_n(6)
-    for i in range(3):
-        bg, eg = g.get_best_result(i)
-        bo, eo = o.get_best_result(i)
-        assert abs(eg - eo) <= 1e-12 * abs(eo)
-        assert rel(bg, bo) <= 1e-12
+
+    def same():
+        for i in range(3):
+            bg, eg = g.get_best_result(i)
+            bo, eo = o.get_best_result(i)
+            assert abs(eg - eo) <= 1e-12 * abs(eo), i
+            assert rel(bg, bo) <= 1e-12, i
+    same()
+    g.reset_best()
+    o.reset_best()
+    g.iterate(4)
+    o.iterate_n(4)
+    same()
+    # m10 = 0.5: x <- 0.5 x + 0.5 P_S x, no projection, get_error() unchanged
+    m = [0, 0, 0, 0, 0, 0, 0, 0, 0, 0.5]
+    g.set_custom_algorithm(*m)
+    o.set_custom_algorithm(*m)
+    g.reset_best()
+    o.reset_best()
+    g.iterate(3)
+    o.iterate_n(3)
+    same()
 
 
 def test_autocorrelation(nd, oracle, synth):
