@@ -18,6 +18,7 @@ sample.  Further legs (each can be skipped):
 Prints one JSON line.  Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
+import dataclasses
 import ctypes
 import importlib
 import json
@@ -319,6 +320,15 @@ def leg_served(nd, synth, count, rate, budget):
     for c in done:
         by.setdefault(c.n, []).append(c.latency * 1e3)
     s["p50_ms_by_size"] = {str(n): round(float(np.percentile(v, 50)), 3) for n, v in sorted(by.items())}
+    # capacity: the first 400 requests of the same mix arriving at once (the
+    # queue drains at the device's pace); sustained iterations/s and jobs/s
+    burst = [dataclasses.replace(r, arrival=0.0) for r in reqs[:400]]
+    srv = serve.Server(be, hbm_budget=budget)
+    done = srv.run(burst)
+    cap = serve.Server.summary(done, srv.elapsed)
+    s["capacity"] = {"requests": len(burst), "elapsed_s": round(srv.elapsed, 3),
+                     "aggregate_iter_s": round(cap["aggregate_iter_s"], 1),
+                     "jobs_per_s": round(len(burst) / srv.elapsed, 1), "peak_active": srv.peak_active}
     return s
 
 
