@@ -2547,9 +2547,21 @@ int nadia_pd_iterate(nadia_pd* d, double* frame_errors) {
       if ((rc = pd_add(d, n))) return rc;
     }
   }
-  if (frame_errors)
-    for (int n = 0; n < nf; n++)
-      if ((rc = fetch_error(d->frames[n].r, &frame_errors[n]))) return rc;
+  if (frame_errors) {
+    // every frame's get_error() with one synchronisation
+    DeviceCtx* c = dev_ctx(d->dev);
+    for (int n = 0; n < nf; n++) {
+      nadia_recon* r = d->frames[n].r;
+      if (r->h_err_valid || !r->last_err) continue;
+      HIPCHK(hipMemcpyAsync(&r->h_err_cache, r->last_err, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int n = 0; n < nf; n++) {
+      nadia_recon* r = d->frames[n].r;
+      if (r->last_err) r->h_err_valid = true;
+      if ((rc = fetch_error(r, &frame_errors[n]))) return rc;
+    }
+  }
   return 0;
 }
 
