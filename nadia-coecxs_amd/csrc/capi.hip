@@ -234,6 +234,11 @@ struct nadia_pd {
   bool weights_set;
   PdFrameDev* dframes;  // device copy of the frame geometry (scale_object)
   int dframes_cap;
+  unsigned char* covered;  // scale_object's covered map of the object, per set of weights
+  long long covered_cap;
+  double* derr;            // the frames' errors gathered on the device, and a pinned host copy
+  double* herr;
+  int err_cap;
   nadia_batch* batch;   // parallel mode: the frames' iterations run as one batch
 };
 
@@ -769,10 +774,11 @@ int autotune(nadia_recon* r) {
 
 // Make W current and the tile variants chosen; upload the descriptor.
 template <typename T>
-int fused_ready(nadia_recon* r) {
+int fused_ready(nadia_recon* r, bool own_desc = true) {
   int rc;
   if ((rc = fused_prepare<T>(r, two_fields(r)))) return rc;
   if ((rc = autotune<T>(r))) return rc;
+  if (!own_desc) return 0;  // a batch launches from its own descriptors
   const JobDesc hj = make_jobdesc(r, r->est->d, r->W, r->errs);
   return upload_jobs(S_(r), r->djob, &hj, 1);
 }
@@ -2058,6 +2064,8 @@ int pd_weights_T(nadia_pd* d) {
   return 0;
 }
 
+PdArgs pd_args(nadia_pd* d, int n);
+
 int pd_weights(nadia_pd* d) {
   if (d->weights_set) return 0;
   d->weights_set = true;
@@ -2082,6 +2090,20 @@ int pd_weights(nadia_pd* d) {
     HIPCHK(hipMemcpyAsync(d->dframes, h.data(), sizeof(PdFrameDev) * nf, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
   }
+  // scale_object's covered map: fixed while the weights and positions are
+  const long long np = (long long)d->nx * d->ny;
+  if (d->covered_cap < np) {
+    if (d->covered) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      (void)hipFree(d->covered);
+      d->covered = nullptr;
+    }
+    if ((rc = alloc((void**)&d->covered, (size_t)np))) return rc;
+    d->covered_cap = np;
+  }
+  const PdArgs a = pd_args(d, -1);
+  DISPATCH(d->prec, k_pd_cover<T>(a, d->dframes, nf, d->covered, c->stream));
+  LAUNCHCHK();
   return 0;
 }
 
@@ -2129,9 +2151,19 @@ int pd_set_result(nadia_pd* d, int n) {
   return 0;
 }
 
-// add_to_object (PhaseDiverseCDI.c++:851-1016)
+// get_result then add_to_object (PhaseDiverseCDI.c++:317-335, 851-1016); at
+// scale 1 without a frame constraint both run as one pass
 int pd_add(nadia_pd* d, int n) {
   int rc;
+  nadia_recon* r = d->frames[n].r;
+  if (d->scale == 1 && !r->tc) {
+    if ((rc = pd_weights(d))) return rc;
+    const PdArgs a = pd_args(d, n);
+    const void* ill = r->kind == NADIA_FRESNEL ? r->Ls : nullptr;
+    DISPATCH(d->prec, k_pd_get_add<T>(a, r->est->d, ill, dev_ctx(d->dev)->stream));
+    LAUNCHCHK();
+    return 0;
+  }
   if ((rc = pd_get_result(d, n)) || (rc = pd_weights(d))) return rc;
   const PdArgs a = pd_args(d, n);
   DISPATCH(d->prec, k_pd_add<T>(a, dev_ctx(d->dev)->stream));
@@ -2139,22 +2171,37 @@ int pd_add(nadia_pd* d, int n) {
   return 0;
 }
 
-// update_from_object (PhaseDiverseCDI.c++:1066-1130)
-int pd_update(nadia_pd* d, int n) {
+// update_from_object then set_result (PhaseDiverseCDI.c++:1066-1130,
+// 317-335) of frames first..first+count-1, kPdSetMax frames per launch
+int pd_update_range(nadia_pd* d, int first, int count) {
   int rc;
   if ((rc = pd_weights(d))) return rc;
-  const PdArgs a = pd_args(d, n);
-  DISPATCH(d->prec, k_pd_update<T>(a, dev_ctx(d->dev)->stream));
-  LAUNCHCHK();
-  return pd_set_result(d, n);
+  const PdArgs a = pd_args(d, -1);
+  for (int k0 = first; k0 < first + count; k0 += kPdSetMax) {
+    PdSetList l{};
+    l.n = std::min(kPdSetMax, first + count - k0);
+    for (int k = 0; k < l.n; k++) {
+      const PdFrame& f = d->frames[k0 + k];
+      l.f[k] = PdFrameDev{f.weight, f.r->nx, f.r->ny, f.x, f.y};
+      l.small[k] = f.result;
+      l.est[k] = f.r->est->d;
+      l.ill[k] = f.r->kind == NADIA_FRESNEL ? f.r->Ls : nullptr;
+      f.r->est->version++;
+    }
+    DISPATCH(d->prec, k_pd_update_set<T>(a, l, dev_ctx(d->dev)->stream));
+    LAUNCHCHK();
+  }
+  return 0;
 }
+
+int pd_update(nadia_pd* d, int n) { return pd_update_range(d, n, 1); }
 
 // scale_object (PhaseDiverseCDI.c++:270-313)
 int pd_scale(nadia_pd* d, double factor) {
   int rc;
   if ((rc = pd_weights(d))) return rc;
   const PdArgs a = pd_args(d, -1);
-  DISPATCH(d->prec, k_pd_scale<T>(a, d->dframes, (int)d->frames.size(), factor, dev_ctx(d->dev)->stream));
+  DISPATCH(d->prec, k_pd_scale<T>(a, d->dframes, (int)d->frames.size(), d->covered, factor, dev_ctx(d->dev)->stream));
   LAUNCHCHK();
   return 0;
 }
@@ -2329,7 +2376,7 @@ int pd_adjust_T(nadia_pd* d, int type, bool forward, int min_x, int max_x, int m
   d->object = temp_object;
   {
     const PdArgs a = pd_args(d, -1);
-    k_pd_scale<T>(a, nullptr, 0, 0.0, c->stream);  // every pixel 1 + 0i
+    k_pd_scale<T>(a, nullptr, 0, nullptr, 0.0, c->stream);  // every pixel 1 + 0i
   }
   d->beta = 1.0;
   d->weights_set = false;
@@ -2423,6 +2470,9 @@ int nadia_pd_destroy(nadia_pd* d) {
   }
   if (d->object) (void)hipFree(d->object);
   if (d->dframes) (void)hipFree(d->dframes);
+  if (d->covered) (void)hipFree(d->covered);
+  if (d->derr) (void)hipFree(d->derr);
+  if (d->herr) (void)hipHostFree(d->herr);
   if (d->batch) nadia_batch_destroy(d->batch);
   delete d;
   return 0;
@@ -2512,7 +2562,7 @@ int nadia_pd_initialise_estimate(nadia_pd* d) {
   if (!d || !d->object) return fail(NADIA_ESTATE, "no frames");
   const PdArgs a = pd_args(d, -1);
   // object = 1 + 0i: scale_object with no frame covering any pixel
-  DISPATCH(d->prec, k_pd_scale<T>(a, nullptr, 0, 0.0, dev_ctx(d->dev)->stream));
+  DISPATCH(d->prec, k_pd_scale<T>(a, nullptr, 0, nullptr, 0.0, dev_ctx(d->dev)->stream));
   LAUNCHCHK();
   int rc;
   for (int n = 0; n < (int)d->frames.size(); n++)
@@ -2527,8 +2577,7 @@ int nadia_pd_iterate(nadia_pd* d, double* frame_errors) {
   const int nf = (int)d->frames.size();
   int rc;
   if (d->parallel) {
-    for (int n = 0; n < nf; n++)
-      if ((rc = pd_update(d, n))) return rc;
+    if ((rc = pd_update_range(d, 0, nf))) return rc;
     if (d->ipc > 0) {
       if (!d->batch) {
         if ((rc = nadia_batch_create(&d->batch))) return rc;
@@ -2548,19 +2597,45 @@ int nadia_pd_iterate(nadia_pd* d, double* frame_errors) {
     }
   }
   if (frame_errors) {
-    // every frame's get_error() with one synchronisation
+    // every frame's get_error(): gathered on the device, one copy into
+    // pinned memory, one synchronisation
     DeviceCtx* c = dev_ctx(d->dev);
+    if (d->err_cap < nf) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (d->derr) (void)hipFree(d->derr);
+      if (d->herr) (void)hipHostFree(d->herr);
+      d->derr = nullptr;
+      d->herr = nullptr;
+      d->err_cap = 0;
+      if ((rc = alloc((void**)&d->derr, sizeof(double) * nf))) return rc;
+      HIPCHK(hipHostMalloc((void**)&d->herr, sizeof(double) * nf, hipHostMallocDefault));
+      d->err_cap = nf;
+    }
+    CopyList l{};
+    std::vector<int> which;
     for (int n = 0; n < nf; n++) {
       nadia_recon* r = d->frames[n].r;
       if (r->h_err_valid || !r->last_err) continue;
-      HIPCHK(hipMemcpyAsync(&r->h_err_cache, r->last_err, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      l.src[l.n] = r->last_err;
+      l.dst[l.n] = d->derr + which.size();
+      which.push_back(n);
+      if (++l.n == kCopyMax) {
+        k_copy_doubles(l, c->stream);
+        l.n = 0;
+      }
     }
+    k_copy_doubles(l, c->stream);
+    LAUNCHCHK();
+    if (!which.empty())
+      HIPCHK(hipMemcpyAsync(d->herr, d->derr, sizeof(double) * which.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (int n = 0; n < nf; n++) {
-      nadia_recon* r = d->frames[n].r;
-      if (r->last_err) r->h_err_valid = true;
-      if ((rc = fetch_error(r, &frame_errors[n]))) return rc;
+    for (size_t q = 0; q < which.size(); q++) {
+      nadia_recon* r = d->frames[which[q]].r;
+      r->h_err_cache = d->herr[q];
+      r->h_err_valid = true;
     }
+    for (int n = 0; n < nf; n++)
+      if ((rc = fetch_error(d->frames[n].r, &frame_errors[n]))) return rc;
   }
   return 0;
 }
@@ -2875,7 +2950,7 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       std::stable_sort(kv.second.begin(), kv.second.end(), [&](int x, int y) { return rem[x] > rem[y]; });
       for (int j : kv.second) {
         nadia_recon* r = b->jobs[j];
-        DISPATCH(r->prec, rc = fused_ready<T>(r));
+        DISPATCH(r->prec, rc = fused_ready<T>(r, false));
         if (rc) return rc;
       }
       nadia_recon* r0 = b->jobs[kv.second[0]];
@@ -2948,13 +3023,23 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
     done += chunk;
     if (done < nmax) HIPCHK(hipStreamSynchronize(c->stream));
   }
+  // keep get_error() valid after the rings are reused: one launch per
+  // kCopyMax jobs
+  CopyList l{};
   for (int j = 0; j < nj; ++j) {
     nadia_recon* r = b->jobs[j];
     if (counts[j] > 0 && r->last_err && r->last_err != r->scal + 8) {
-      HIPCHK(hipMemcpyAsync(r->scal + 8, r->last_err, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      l.src[l.n] = r->last_err;
+      l.dst[l.n] = r->scal + 8;
       r->last_err = r->scal + 8;
+      if (++l.n == kCopyMax) {
+        k_copy_doubles(l, c->stream);
+        l.n = 0;
+      }
     }
   }
+  k_copy_doubles(l, c->stream);
+  LAUNCHCHK();
   return 0;
 }
 

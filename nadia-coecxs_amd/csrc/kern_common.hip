@@ -133,6 +133,15 @@ void k_xc_final(const XcBest* partials, int n, XcBest* out, hipStream_t st) {
   hipLaunchKernelGGL(xc_final_kernel, dim3(1), dim3(64), 0, st, partials, n, out);
 }
 
+__global__ void copy_doubles_kernel(CopyList l) {
+  const int k = threadIdx.x;
+  if (k < l.n) *l.dst[k] = *l.src[k];
+}
+
+void k_copy_doubles(const CopyList& l, hipStream_t st) {
+  if (l.n > 0) hipLaunchKernelGGL(copy_doubles_kernel, dim3(1), dim3(kCopyMax), 0, st, l);
+}
+
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st) {
   hipLaunchKernelGGL(final_sum2_kernel, dim3(1), dim3(256), 0, st, partials, n, out_ratio, out_pair);
 }

@@ -116,6 +116,24 @@ struct PdArgs {
   void* small;         // the frame's result (lnx x lny complex)
 };
 
+// frames of one update-and-set launch, passed by value (frame k = blockIdx.y)
+constexpr int kPdSetMax = 16;
+struct PdSetList {
+  PdFrameDev f[kPdSetMax];
+  void* small[kPdSetMax];      // result, lnx x lny complex
+  void* est[kPdSetMax];        // the frame's estimate
+  const void* ill[kPdSetMax];  // illumination at the sample (Fresnel), else null
+  int n;
+};
+
+// up to kCopyMax single doubles copied device to device in one launch
+constexpr int kCopyMax = 32;
+struct CopyList {
+  const double* src[kCopyMax];
+  double* dst[kCopyMax];
+  int n;
+};
+
 // element-wise and reduction kernels (kern_misc.inc)
 template <typename T> void k_invert(void* buf, int nx, int ny, int scale, hipStream_t st);
 template <typename T> void k_scale_intensity(void* buf, const void* q, int nx, int ny, double* partials, int nblocks, hipStream_t st);
@@ -148,8 +166,12 @@ template <typename T> void k_tc_apply(void* t, const TcDev& d, long long n, hipS
 // PhaseDiverseCDI add_to_object / update_from_object / scale_object
 template <typename T> void k_pd_add(const PdArgs& a, hipStream_t st);
 template <typename T> void k_pd_update(const PdArgs& a, hipStream_t st);
-template <typename T> void k_pd_scale(const PdArgs& a, const PdFrameDev* frames, int nframes, double factor,
+template <typename T> void k_pd_update_set(const PdArgs& a, const PdSetList& l, hipStream_t st);
+template <typename T> void k_pd_get_add(const PdArgs& a, const void* est, const void* ill, hipStream_t st);
+template <typename T> void k_pd_cover(const PdArgs& a, const PdFrameDev* frames, int nframes, unsigned char* covered,
                                       hipStream_t st);
+template <typename T> void k_pd_scale(const PdArgs& a, const PdFrameDev* frames, int nframes,
+                                      const unsigned char* covered, double factor, hipStream_t st);
 // PhaseDiverseCDI::adjust_positions (NADIA/src/PhaseDiverseCDI.c++:337-579):
 // the 1 - |T| images of the global sub-grid around a frame and of the frame
 // (cross-correlation), the object sub-grid (minimum error), bilinear
@@ -188,6 +210,8 @@ struct XcScan {
 // best ratio (first in index order on ties) over kEwBlocks partials, then one
 template <typename T> void k_xc_scan(const XcScan& a, XcBest* partials, hipStream_t st);
 void k_xc_final(const XcBest* partials, int n, XcBest* out, hipStream_t st);
+// *dst[k] = *src[k] for the l.n pairs, one launch
+void k_copy_doubles(const CopyList& l, hipStream_t st);
 // update_n_best on the device: rank the error at `err`, then copy `bytes` of
 // the estimate `src` into bufs[new rank's buffer] when it placed
 void k_nbest(const double* err, double* best_err, int* perm, int n, int* target, const void* src, void* const* bufs,

@@ -56,9 +56,10 @@ def test_planar_frames(nd, oracle, synth, parallel, scale):
         assert rel(est.get(), xo) <= 1e-10
 
 
-def test_fresnel_frames(nd, oracle, synth):
-    pg = nd.PhaseDiverseCDI(beta=0.5, gamma=1.5, parallel=True)
-    po = oracle.OraclePhaseDiverse(beta=0.5, gamma=1.5, parallel=True)
+@pytest.mark.parametrize("parallel", [False, True], ids=["series", "parallel"])
+def test_fresnel_frames(nd, oracle, synth, parallel):
+    pg = nd.PhaseDiverseCDI(beta=0.5, gamma=1.5, parallel=parallel)
+    po = oracle.OraclePhaseDiverse(beta=0.5, gamma=1.5, parallel=parallel)
     keep = []
     for k, (x, y) in enumerate([(0, 0), (6, -4)]):
         d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
@@ -92,3 +93,24 @@ def test_fp32_frames(nd, oracle, synth):
         pg.iterate()
         po.iterate()
     assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-5
+
+
+def test_many_frames(nd, oracle, synth):
+    """More frames than one update launch and one error gather take (16):
+    18 planar frames on a 6 x 3 grid of positions, parallel mode."""
+    pos = [(9.0 * (k % 6) - 20, 7.5 * (k // 6) - 8) for k in range(18)]
+    gpu, cpu = planar_frames(nd, oracle, synth, pos, n=32, alg="HIO")
+    pg = nd.PhaseDiverseCDI(beta=0.9, gamma=1.0, parallel=True)
+    po = oracle.OraclePhaseDiverse(beta=0.9, gamma=1.0, parallel=True)
+    for (est, g), (xo, o), (x, y) in zip(gpu, cpu, pos):
+        pg.add_new_position(g, x, y, 1.0)
+        po.add_new_position(o, x, y, 1.0)
+    pg.initialise_estimate()
+    po.initialise_estimate()
+    for it in range(3):
+        eg = pg.iterate()
+        po.iterate()
+        eo = np.array([o.get_error() for _, o in cpu])
+        assert eg.shape == (18,)
+        assert (np.abs(eg - eo) / eo).max() <= 1e-9, it
+    assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-10
