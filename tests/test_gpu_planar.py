@@ -336,3 +336,36 @@ def test_fp64_2048_repeatable_and_exact(nd, oracle, synth, cv):
     o.set_algorithm(oracle.HIO)
     o.iterate_n(2)
     assert np.abs(runs[0] - xo).max() <= 1e-12 * np.abs(xo).max()
+
+
+@pytest.mark.parametrize("n", [4096, 3072])
+def test_largest_sizes(nd, oracle, synth, n):
+    """The largest lengths per axis (2^12 and 3*2^10) through the fused
+    passes: two HIO iterations in FP64 match the oracle to 1e-12 (max abs
+    over max |x|), FP32 stays within 1e-5 (relative L2) of the FP64 oracle."""
+    d = synth.planar_star(n=n)
+    xo = d["x0"].astype(np.complex128).copy()
+    o = oracle.OracleCDI(xo, oracle.PLANAR)
+    o.set_support(d["support"])
+    o.set_intensity(d["intensity"])
+    o.set_beam_stop(d["beam_stop"])
+    o.set_algorithm(oracle.HIO)
+    o.iterate_n(2)
+    eo = o.get_error()
+    for double in (True, False):
+        cdt = np.complex128 if double else np.complex64
+        est = nd.Complex2D(n, n, double=double, data=d["x0"].astype(cdt))
+        g = nd.PlanarCDI(est)
+        g.set_support(d["support"])
+        g.set_intensity(d["intensity"])
+        g.set_beam_stop(d["beam_stop"])
+        g.set_algorithm(nd.HIO)
+        e = g.iterate(2)
+        x = est.get()
+        if double:
+            assert np.abs(x - xo).max() <= 1e-12 * np.abs(xo).max()
+            assert abs(e[-1] - eo) <= 1e-12 * eo
+        else:
+            assert rel(x.astype(np.complex128), xo) <= 1e-5
+            assert abs(e[-1] - eo) <= 1e-5 * eo
+        del g, est
