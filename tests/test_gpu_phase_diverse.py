@@ -114,3 +114,38 @@ def test_many_frames(nd, oracle, synth):
         assert eg.shape == (18,)
         assert (np.abs(eg - eo) / eo).max() <= 1e-9, it
     assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-10
+
+
+def test_fresnel_frames_with_constraint(nd, oracle, synth):
+    """Frames carrying a TransmissionConstraint: get_result applies it before
+    add_to_object (the two steps stay apart), the frames iterate on the
+    unfused path."""
+    pg = nd.PhaseDiverseCDI(beta=0.7, gamma=1.0, parallel=True)
+    po = oracle.OraclePhaseDiverse(beta=0.7, gamma=1.0, parallel=True)
+    keep = []
+    rng = np.random.default_rng(7)
+    for k, (x, y) in enumerate([(0, 0), (5, -3)]):
+        d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
+        n = est.get().shape[0]
+        tg = nd.TransmissionConstraint(n, n)
+        tcs = oracle.OracleConstraint(n, n)
+        for tc in (tg, tcs):
+            tc.add_complex_constraint(d["support"], 0.5, 0.5)
+        g.set_complex_constraint(tg)
+        o.set_complex_constraint(tcs)
+        # phases away from zero keep the region constant well conditioned
+        T0 = np.exp((0.3 + 1j) * -rng.uniform(0.1, 0.4, (n, n)))
+        g.set_transmission_function(T0)
+        o.set_transmission_function(T0)
+        pg.add_new_position(g, x, y, 1.0)
+        po.add_new_position(o, x, y, 1.0)
+        keep.append((est, g, xo, o, tg, tcs))
+    pg.initialise_estimate()
+    po.initialise_estimate()
+    assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-12
+    for it in range(3):
+        pg.iterate()
+        po.iterate()
+        assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-10, it
+    for est, g, xo, o, tg, tcs in keep:
+        assert rel(est.get(), xo) <= 1e-10
