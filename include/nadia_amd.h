@@ -201,8 +201,9 @@ int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size);
  * apply_constraint on a field.  set_complex_constraint attaches one to a
  * reconstruction (NULL detaches): apply_support then applies it after the
  * support (to the estimate for PlanarCDI, to the transmission function for
- * FresnelCDI) and iterate() takes the unfused path, as the constraint's
- * per-region means need the whole field between projection and update.  The
+ * FresnelCDI).  The constraint's per-region means need the whole field
+ * between projection and update: ER then runs the fused passes with the
+ * constraint between iterations, other algorithms the unfused path.  The
  * constraint is not owned and must outlive its use. */
 int nadia_constraint_create(int nx, int ny, nadia_constraint** out);
 int nadia_constraint_destroy(nadia_constraint* tc);

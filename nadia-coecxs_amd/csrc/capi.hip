@@ -398,9 +398,16 @@ int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec);
 // then the transmission constraint on the estimate (planar) or on its
 // transmission function (Fresnel: get_transmission_function applies it,
 // set_transmission_function writes the exit-surface wave back).
+int op_constraint(nadia_recon* r, void* buf);
+
 int op_support(nadia_recon* r, void* buf) {
   DISPATCH(r->prec, k_support<T>(buf, r->S, r->npix, S_(r)));
   LAUNCHCHK();
+  return op_constraint(r, buf);
+}
+
+// the transmission-constraint part of apply_support
+int op_constraint(nadia_recon* r, void* buf) {
   if (!r->tc) return 0;
   if (r->kind == NADIA_FRESNEL) {
     int rc;
@@ -475,8 +482,16 @@ bool has_projection(const nadia_recon* r) {
 }
 
 // a transmission constraint needs whole-field reductions between the
-// projection and the update, so it runs the unfused path
+// projection and the update, so a constrained job does not share the fused
+// launches (ER takes fused_tc_ok's path, the rest the unfused path)
 bool fused_ok(const nadia_recon* r) { return r->path == NADIA_PATH_FUSED && has_projection(r) && !r->tc; }
+
+// ER with a transmission constraint: the fused passes give support(P_F x),
+// the constraint then runs on the new estimate between iterations
+bool fused_tc_ok(const nadia_recon* r) {
+  return r->path == NADIA_PATH_FUSED && has_projection(r) && r->tc && r->algorithm == NADIA_ER &&
+         r->kind != NADIA_FRESNEL_WF;
+}
 
 struct Ev {
   hipEvent_t a, b;
@@ -812,6 +827,34 @@ int iterate_fused(nadia_recon* r, int n) {
   }
   fused_done(r, n);
   return collect_events(r, evs);
+}
+
+// ER iterations with a transmission constraint (BaseCDI::iterate's ER
+// branch, NADIA/src/BaseCDI.c++:303-308, with FresnelCDI::apply_support,
+// NADIA/src/FresnelCDI.c++:203-215): the fused column and row passes give
+// x = support(P_F x); the constraint then runs on x as in op_support, and
+// the row transform W is rebuilt from the constrained x for the next
+// column pass.
+// One such iteration, its error into ring slot k (after fused_ready).
+template <typename T>
+int fused_tc_step(nadia_recon* r, int k) {
+  int rc;
+  if ((rc = launch_fused<T>(r->ctx, group_key(r), r->djob, 1, k, r->col_v, r->row_v, r, nullptr))) return rc;
+  if ((rc = op_constraint(r, r->est->d))) return rc;
+  r->w_valid = false;
+  if ((rc = fused_prepare<T>(r, false))) return rc;
+  if (wants_n_best(r) && (rc = update_n_best(r, r->errs + k, S_(r)))) return rc;
+  return 0;
+}
+
+template <typename T>
+int iterate_fused_tc(nadia_recon* r, int n) {
+  int rc;
+  if ((rc = fused_ready<T>(r))) return rc;
+  for (int k = 0; k < n; ++k)
+    if ((rc = fused_tc_step<T>(r, k))) return rc;
+  fused_done(r, n);
+  return 0;
 }
 
 int fetch_error(nadia_recon* r, double* out) {
@@ -1312,6 +1355,9 @@ int nadia_recon_iterate(nadia_recon* r, int n, double* errors) {
     if (chunk > r->ring_cap) chunk = r->ring_cap;
     if (fused_ok(r)) {
       DISPATCH(r->prec, rc = iterate_fused<T>(r, chunk));
+      if (rc) return rc;
+    } else if (fused_tc_ok(r)) {
+      DISPATCH(r->prec, rc = iterate_fused_tc<T>(r, chunk));
       if (rc) return rc;
     } else {
       for (int k = 0; k < chunk; ++k) {
@@ -2999,10 +3045,23 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
         if (rc || (rc = L.n_best_after(a, k - eager_from, s2))) return rc;
       }
     }
+    // jobs outside the shared launches: ER with a transmission constraint
+    // takes its own fused iterations (as when it runs alone), the rest the
+    // unfused path
+    for (int j : plain)
+      if (fused_tc_ok(b->jobs[j])) {
+        DISPATCH(b->jobs[j]->prec, rc = fused_ready<T>(b->jobs[j]));
+        if (rc) return rc;
+      }
     for (int k = eager_from; k < chunk; ++k)
       for (int j : plain) {
         nadia_recon* r = b->jobs[j];
         if (rem[j] <= k) continue;
+        if (fused_tc_ok(r)) {
+          DISPATCH(r->prec, rc = fused_tc_step<T>(r, k - eager_from));
+          if (rc) return rc;
+          continue;
+        }
         if ((rc = iterate_plain(r, r->errs + (k - eager_from)))) return rc;
         if (wants_n_best(r) && (rc = update_n_best(r, current_error_slot(r, r->errs + (k - eager_from)), c->stream)))
           return rc;
@@ -3018,6 +3077,7 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       if (rem[j] == 0) continue;
       nadia_recon* r = b->jobs[j];
       if (fused_ok(r)) fused_done(r, graph ? last_slot + 1 : rem[j]);
+      else if (fused_tc_ok(r)) fused_done(r, rem[j]);
       else r->est->version++;
     }
     done += chunk;

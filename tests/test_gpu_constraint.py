@@ -1,6 +1,8 @@
 """GPU parity of TransmissionConstraint (nadia_constraint) against the CPU
 oracle: the constraint alone on a field, and inside apply_support during
-PlanarCDI and FresnelCDI iterations (which then take the unfused path)."""
+PlanarCDI and FresnelCDI iterations: ER on the fused passes with the
+constraint between iterations, other algorithms and the plain path
+unfused."""
 import numpy as np
 import pytest
 
@@ -68,12 +70,14 @@ def test_custom_constraint_runs_after_builtin(nd, oracle):
     assert rel(f.get(), 0.5 * to) <= 1e-12
 
 
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
 @pytest.mark.parametrize("alg", ["ER", "HIO"])
-def test_planar_iterations_with_constraint(nd, oracle, synth, alg):
+def test_planar_iterations_with_constraint(nd, oracle, synth, alg, path):
     n = 128
     d = synth.planar_block(n=n, block=48, support=56)
     est = nd.Complex2D(n, n, data=d["x0"])
     g = nd.PlanarCDI(est)
+    g.set_path(path)
     xo = d["x0"].astype(np.complex128).copy()
     o = oracle.OracleCDI(xo, oracle.PLANAR)
     # two overlapping regions inside the support (log|x| is finite there)
@@ -96,8 +100,30 @@ def test_planar_iterations_with_constraint(nd, oracle, synth, alg):
     assert abs(tg.c_mean(0) - to.c_mean(0)) <= 1e-10 * abs(to.c_mean(0))
 
 
-def test_fresnel_iterations_with_constraint(nd, oracle, synth):
-    d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
+def fresnel_constrained(nd, oracle, synth, double=True, path=0):
+    """A Fresnel pair with one complex-constraint region over the support,
+    started from a transmission function with phases well away from zero:
+    initialise_estimate's real T makes the region constant c = sum|log|T|| /
+    sum|arg T| divide by rounding-level phases (ill-conditioned on any two
+    implementations)."""
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth, double=double, path=path)
+    n = 128
+    tg = nd.TransmissionConstraint(n, n)
+    tcs = oracle.OracleConstraint(n, n)
+    for tc in (tg, tcs):
+        tc.add_complex_constraint(d["support"], 0.5, 0.5)
+    g.set_complex_constraint(tg)
+    o.set_complex_constraint(tcs)
+    rng = np.random.default_rng(5)
+    T0 = np.exp((0.3 + 1j) * -rng.uniform(0.1, 0.4, (n, n)))
+    g.set_transmission_function(T0)
+    o.set_transmission_function(T0)
+    return est, g, xo, o, tg, tcs
+
+
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+def test_fresnel_iterations_with_constraint(nd, oracle, synth, path):
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth, path=path)
     n = 128
     tg = nd.TransmissionConstraint(n, n)
     tcs = oracle.OracleConstraint(n, n)
@@ -122,3 +148,37 @@ def test_fresnel_iterations_with_constraint(nd, oracle, synth):
     assert rel(est.get(), xo) <= 1e-10
     # get_transmission_function applies the constraint, as in the reference
     assert rel(g.get_transmission_function(), o.get_transmission_function()) <= 1e-10
+
+
+def test_fresnel_constraint_fp32(nd, oracle, synth):
+    """FP32 (fused ER with the constraint) within the Fresnel FP32 bound
+    (1e-4) of the FP64 oracle."""
+    est, g, xo, o, tg, tcs = fresnel_constrained(nd, oracle, synth, double=False)
+    eg = g.iterate(4)
+    eo = o.iterate_n(4)
+    assert (np.abs(eg - eo) / eo).max() <= 1e-4
+    assert rel(est.get().astype(np.complex128), xo) <= 1e-4
+
+
+def test_constrained_job_in_batch_matches_alone(nd, oracle, synth):
+    """A constrained Fresnel ER job batched with an unconstrained planar job:
+    both bit-identical to running alone."""
+    est_a, g_a, _, _, tg_a, _ = fresnel_constrained(nd, oracle, synth)
+    est_b, g_b, _, _, tg_b, _ = fresnel_constrained(nd, oracle, synth)
+    e_a = g_a.iterate(5)
+    d = synth.planar_block(n=64, block=24, support=30)
+    est_p = nd.Complex2D(64, 64, data=d["x0"])
+    p = nd.PlanarCDI(est_p)
+    p.set_support(d["support"])
+    p.set_intensity(d["intensity"])
+    p.set_algorithm(nd.HIO)
+    est_q = nd.Complex2D(64, 64, data=d["x0"])
+    q = nd.PlanarCDI(est_q)
+    q.set_support(d["support"])
+    q.set_intensity(d["intensity"])
+    q.set_algorithm(nd.HIO)
+    e_q = q.iterate(5)
+    b = nd.Batch([g_b, p])
+    e = b.iterate(5)
+    assert np.array_equal(e[:, 0], e_a) and np.array_equal(e[:, 1], e_q)
+    assert np.array_equal(est_b.get(), est_a.get()) and np.array_equal(est_p.get(), est_q.get())
