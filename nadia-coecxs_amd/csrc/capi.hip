@@ -391,7 +391,8 @@ int op_project(nadia_recon* r, void* buf, double* slot) {
   return op_from_detector(r, buf);
 }
 
-int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec);
+int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec, const void* ill = nullptr, int nx = 0,
+             int ny = 0);
 
 // BaseCDI::apply_support (NADIA/src/BaseCDI.c++:130-134) and
 // FresnelCDI::apply_support (NADIA/src/FresnelCDI.c++:203-215): support,
@@ -410,6 +411,10 @@ int op_support(nadia_recon* r, void* buf) {
 int op_constraint(nadia_recon* r, void* buf) {
   if (!r->tc) return 0;
   if (r->kind == NADIA_FRESNEL) {
+    // without a host function the kernels read T from the exit-surface wave
+    // and write the wave back themselves (the same roundings as the three
+    // steps below)
+    if (!r->tc->custom) return tc_apply(r->tc, S_(r), buf, r->prec, r->Ls, r->nx, r->ny);
     int rc;
     if (!r->ttmp && (rc = alloc(&r->ttmp, r->npix * r->esz))) return rc;
     DISPATCH(r->prec, k_trans_from_esw<T>(buf, r->Ls, r->ttmp, r->npix, S_(r)));
@@ -1831,17 +1836,27 @@ TcDev tc_dev(const nadia_constraint* tc) {
 }
 
 // TransmissionConstraint::apply_constraint (NADIA/src/TransmissionConstraint.c++:81-172)
-int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec) {
+// on buf, or (ill set, no host function) on the transmission function of the
+// exit-surface wave buf holds, written back as the wave
+int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec, const void* ill, int nx, int ny) {
   int rc;
   if ((rc = tc_upload(tc, st))) return rc;
   const long long n = (long long)tc->nx * tc->ny;
-  const TcDev d = tc_dev(tc);
+  TcDev d = tc_dev(tc);
+  if (ill) {
+    if (tc->custom) return fail(NADIA_EINVAL, "a host constraint function needs the transmission function array");
+    d.ill = ill;
+    d.nx = nx;
+    d.ny = ny;
+  }
   if (tc->nreg > 0) {
     DISPATCH(prec, k_tc_stats<T>(buf, d, n, tc->partials, st));
     k_tc_final(d, tc->partials, kEwBlocks, st);
     LAUNCHCHK();
   }
-  if (tc->nreg > 0 || tc->charge_flip || tc->unity) {
+  // with ill the pass also does the T -> ESW round trip the reference does
+  // even when nothing changes T
+  if (tc->nreg > 0 || tc->charge_flip || tc->unity || ill) {
     DISPATCH(prec, k_tc_apply<T>(buf, d, n, st));
     LAUNCHCHK();
   }

@@ -97,6 +97,11 @@ struct TcDev {
   double* c_mean;            // per region; recomputed unless fixed
   const int* fixed;
   int charge_flip, flip_sign, unity;
+  // FresnelCDI: the buffer holds the exit-surface wave; the constraint acts
+  // on T = (ESW + L)/L (L = ill, the illumination at the sample) and writes
+  // ESW = (T - 1) L back in the columns j < nx (set_transmission_function)
+  const void* ill;
+  int nx, ny;
 };
 
 // PhaseDiverseCDI frame geometry (NADIA/src/PhaseDiverseCDI.c++): local

@@ -182,3 +182,24 @@ def test_constrained_job_in_batch_matches_alone(nd, oracle, synth):
     e = b.iterate(5)
     assert np.array_equal(e[:, 0], e_a) and np.array_equal(e[:, 1], e_q)
     assert np.array_equal(est_b.get(), est_a.get()) and np.array_equal(est_p.get(), est_q.get())
+
+
+def test_fresnel_empty_constraint_round_trip(nd, oracle, synth):
+    """A constraint with no region, no charge flipping and no unity cap still
+    takes the exit wave through T and back (get_transmission_function then
+    set_transmission_function), as the reference does."""
+    d, est, g, xo, o = fresnel_pair(nd, oracle, synth)
+    n = 128
+    tg = nd.TransmissionConstraint(n, n)
+    tcs = oracle.OracleConstraint(n, n)
+    for tc in (tg, tcs):
+        tc.set_charge_flipping(False)
+        tc.set_enforce_unity(False)
+    g.set_complex_constraint(tg)
+    o.set_complex_constraint(tcs)
+    g.initialise_estimate(3)
+    o.initialise_estimate(3)
+    eg = g.iterate(5)
+    eo = o.iterate_n(5)
+    assert (np.abs(eg - eo) / eo).max() <= 1e-10
+    assert rel(est.get(), xo) <= 1e-10
