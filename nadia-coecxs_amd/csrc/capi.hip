@@ -247,6 +247,10 @@ struct nadia_batch {
   std::vector<nadia_recon*> jobs;
   JobDesc* djobs;  // device descriptors, grouped
   int cap;
+  JobDesc* hjobs;  // pinned staging of the descriptors; hev marks the last copy from it
+  int hcap;
+  hipEvent_t hev;
+  bool hev_pending;
   // graph mode: uniform sweeps run as replays of a captured graph of
   // `gsteps` iterations; gsig describes the launches it was captured for
   int graph_steps;
@@ -533,27 +537,36 @@ int collect_events(nadia_recon* r, std::vector<Ev>& evs) {
   return 0;
 }
 
+// the row transform that makes W from the estimate (the fused passes' state
+// between iterations)
+RowArgs w_rebuild_args(const nadia_recon* r, const void* twy) {
+  RowArgs ra{};
+  ra.mode = ROW_FFT;
+  ra.nrows = r->nx;
+  ra.ny = r->ny;
+  ra.tw = twy;
+  ra.scale = 1.0;
+  if (r->kind == NADIA_FRESNEL_WF) {
+    ra.dir = +1;
+    ra.pre_sign = 1;
+  } else {
+    ra.dir = -1;
+  }
+  return ra;
+}
+
+bool w_stale(const nadia_recon* r) { return !r->w_valid || r->w_version != r->est->version; }
+
 template <typename T>
 int fused_prepare(nadia_recon* r, bool two) {
   void* twy = twiddles(r->ctx, r->ny, r->prec, true);
   if (!twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
   const long long n = r->npix;
   const int rv = r->row_v < 0 ? 0 : r->row_v;
-  if (!r->w_valid || r->w_version != r->est->version) {
-    RowArgs ra{};
-    ra.mode = ROW_FFT;
-    ra.nrows = r->nx;
-    ra.ny = r->ny;
-    ra.tw = twy;
+  if (w_stale(r)) {
+    RowArgs ra = w_rebuild_args(r, twy);
     ra.src = r->est->d;
     ra.dst = r->W;
-    ra.scale = 1.0;
-    if (r->kind == NADIA_FRESNEL_WF) {
-      ra.dir = +1;
-      ra.pre_sign = 1;
-    } else {
-      ra.dir = -1;
-    }
     launch_row<T>(ra, 1, rv, S_(r));
     LAUNCHCHK();
     r->w_valid = true;
@@ -2799,6 +2812,8 @@ int nadia_batch_destroy(nadia_batch* b) {
   if (c) (void)hipStreamSynchronize(c->stream);
   if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
   if (b->djobs) (void)hipFree(b->djobs);
+  if (b->hjobs) (void)hipHostFree(b->hjobs);
+  if (b->hev) (void)hipEventDestroy(b->hev);
   delete b;
   return 0;
 }
@@ -2854,6 +2869,7 @@ struct BatchLaunch {
   int cv, rv;
   std::vector<int> rem;
   std::vector<nadia_recon*> jobs;  // in descriptor order
+  bool rebuild_w = false;          // every job's W is rebuilt in one launch first
   // after iteration k (ring slot `slot`) of the first a jobs: their n-best
   // bookkeeping, on the group's stream
   int n_best_after(int a, int slot, hipStream_t st) const {
@@ -3009,13 +3025,23 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
     host.reserve(nj);
     for (auto& kv : groups) {
       std::stable_sort(kv.second.begin(), kv.second.end(), [&](int x, int y) { return rem[x] > rem[y]; });
+      // every job's estimate changed since its last iteration (e.g. frames
+      // updated from a PhaseDiverseCDI object): one W rebuild for the group
+      // after the descriptors are up, instead of one launch per job
+      bool all_stale = kv.second.size() > 1;
+      for (int j : kv.second) all_stale = all_stale && w_stale(b->jobs[j]);
       for (int j : kv.second) {
         nadia_recon* r = b->jobs[j];
+        if (all_stale) {
+          r->w_valid = true;
+          r->w_version = r->est->version;
+        }
         DISPATCH(r->prec, rc = fused_ready<T>(r, false));
         if (rc) return rc;
       }
       nadia_recon* r0 = b->jobs[kv.second[0]];
       BatchLaunch L{kv.first, b->djobs + host.size(), r0->col_v, r0->row_v, {}, {}};
+      L.rebuild_w = all_stale;
       for (int j : kv.second) {
         host.push_back(make_jobdesc(b->jobs[j], b->jobs[j]->est->d, b->jobs[j]->W, b->jobs[j]->errs));
         L.rem.push_back(rem[j]);
@@ -3023,7 +3049,35 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       }
       launches.push_back(std::move(L));
     }
-    if (!host.empty() && (rc = upload_jobs(c->stream, b->djobs, host.data(), (int)host.size()))) return rc;
+    if (!host.empty()) {
+      // through pinned staging: the copy is asynchronous, and the staging is
+      // reused only after the previous copy from it has completed
+      if (b->hcap < (int)host.size()) {
+        if (b->hev_pending) HIPCHK(hipEventSynchronize(b->hev));
+        if (b->hjobs) (void)hipHostFree(b->hjobs);
+        b->hjobs = nullptr;
+        b->hcap = 0;
+        HIPCHK(hipHostMalloc((void**)&b->hjobs, sizeof(JobDesc) * host.size(), hipHostMallocDefault));
+        b->hcap = (int)host.size();
+      } else if (b->hev_pending) {
+        HIPCHK(hipEventSynchronize(b->hev));
+      }
+      if (!b->hev) HIPCHK(hipEventCreateWithFlags(&b->hev, hipEventDisableTiming));
+      std::memcpy(b->hjobs, host.data(), sizeof(JobDesc) * host.size());
+      HIPCHK(hipMemcpyAsync(b->djobs, b->hjobs, sizeof(JobDesc) * host.size(), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipEventRecord(b->hev, c->stream));
+      b->hev_pending = true;
+    }
+    for (const BatchLaunch& L : launches) {
+      if (!L.rebuild_w) continue;
+      nadia_recon* r0 = L.jobs[0];
+      void* twy = twiddles(c, r0->ny, r0->prec, true);
+      if (!twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
+      RowArgs ra = w_rebuild_args(r0, twy);
+      ra.jobs = L.d;
+      DISPATCH(r0->prec, launch_row<T>(ra, (int)L.jobs.size(), L.rv, c->stream));
+      LAUNCHCHK();
+    }
     // graph replays for the uniform, fused-only part; the rest eagerly
     const int G = b->graph_steps;
     const bool graph = G > 0 && uniform && plain.empty() && !launches.empty() && chunk >= G;

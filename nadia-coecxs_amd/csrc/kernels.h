@@ -45,7 +45,7 @@ struct RowArgs {
   int iter;             // error ring slot written by this launch
   int nrows, ny;        // rows per job, row length
   const void* tw;       // twiddle table of length ny (sign -1 values)
-  // ROW_FFT (one field, no job table)
+  // ROW_FFT: one field src -> dst, or (jobs set) every job's x -> w
   const void* src;
   void* dst;
   int dir;
