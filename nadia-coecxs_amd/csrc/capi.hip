@@ -198,6 +198,8 @@ struct nadia_recon {
   // job descriptors of the fused passes (device): [0] this reconstruction,
   // [1] scratch copy used by the autotuner
   JobDesc* djob;
+  JobDesc djob_host;  // what djob[0] holds (valid when djob_set)
+  bool djob_set;
   double* tune_errs;
 };
 
@@ -813,6 +815,10 @@ int fused_ready(nadia_recon* r, bool own_desc = true) {
   if ((rc = autotune<T>(r))) return rc;
   if (!own_desc) return 0;  // a batch launches from its own descriptors
   const JobDesc hj = make_jobdesc(r, r->est->d, r->W, r->errs);
+  // unchanged since the last upload (a loop of single iterations): no copy
+  if (r->djob_set && std::memcmp(&hj, &r->djob_host, sizeof(JobDesc)) == 0) return 0;
+  r->djob_host = hj;
+  r->djob_set = true;
   return upload_jobs(S_(r), r->djob, &hj, 1);
 }
 
