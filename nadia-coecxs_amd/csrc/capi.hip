@@ -399,6 +399,8 @@ int op_project(nadia_recon* r, void* buf, double* slot) {
 
 int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec, const void* ill = nullptr, int nx = 0,
              int ny = 0);
+int tc_stats(nadia_constraint* tc, hipStream_t st, const void* buf, int prec, const void* ill, int nx, int ny,
+             TcDev* out);
 
 // BaseCDI::apply_support (NADIA/src/BaseCDI.c++:130-134) and
 // FresnelCDI::apply_support (NADIA/src/FresnelCDI.c++:203-215): support,
@@ -859,14 +861,33 @@ int iterate_fused(nadia_recon* r, int n) {
 // x = support(P_F x); the constraint then runs on x as in op_support, and
 // the row transform W is rebuilt from the constrained x for the next
 // column pass.
-// One such iteration, its error into ring slot k (after fused_ready).
+// One such iteration, its error into ring slot k (after fused_ready): the
+// region means, then one row pass applying the constraint and rebuilding W
+// (a host constraint function takes the separate steps).
 template <typename T>
 int fused_tc_step(nadia_recon* r, int k) {
   int rc;
   if ((rc = launch_fused<T>(r->ctx, group_key(r), r->djob, 1, k, r->col_v, r->row_v, r, nullptr))) return rc;
-  if ((rc = op_constraint(r, r->est->d))) return rc;
-  r->w_valid = false;
-  if ((rc = fused_prepare<T>(r, false))) return rc;
+  if (r->tc->custom) {
+    if ((rc = op_constraint(r, r->est->d))) return rc;
+    r->w_valid = false;
+    if ((rc = fused_prepare<T>(r, false))) return rc;
+  } else {
+    const bool fres = r->kind == NADIA_FRESNEL;
+    TcDev d{};
+    if ((rc = tc_stats(r->tc, S_(r), r->est->d, r->prec, fres ? r->Ls : nullptr, r->nx, r->ny, &d))) return rc;
+    void* twy = twiddles(r->ctx, r->ny, r->prec, true);
+    if (!twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
+    RowArgs ra = w_rebuild_args(r, twy);
+    ra.mode = ROW_TC;
+    ra.src = r->est->d;
+    ra.dst = r->W;
+    ra.tc = d;
+    launch_row<T>(ra, 1, r->row_v < 0 ? 0 : r->row_v, S_(r));
+    LAUNCHCHK();
+    r->w_valid = true;
+    r->w_version = r->est->version;
+  }
   if (wants_n_best(r) && (rc = update_n_best(r, r->errs + k, S_(r)))) return rc;
   return 0;
 }
@@ -1854,10 +1875,11 @@ TcDev tc_dev(const nadia_constraint* tc) {
   return d;
 }
 
-// TransmissionConstraint::apply_constraint (NADIA/src/TransmissionConstraint.c++:81-172)
-// on buf, or (ill set, no host function) on the transmission function of the
-// exit-surface wave buf holds, written back as the wave
-int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec, const void* ill, int nx, int ny) {
+// The region means of apply_constraint (TransmissionConstraint.c++:81-120)
+// over buf, or (ill set) over the transmission function of the exit-surface
+// wave buf holds; *out receives the device view for the apply step.
+int tc_stats(nadia_constraint* tc, hipStream_t st, const void* buf, int prec, const void* ill, int nx, int ny,
+             TcDev* out) {
   int rc;
   if ((rc = tc_upload(tc, st))) return rc;
   const long long n = (long long)tc->nx * tc->ny;
@@ -1873,6 +1895,18 @@ int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec, const vo
     k_tc_final(d, tc->partials, kEwBlocks, st);
     LAUNCHCHK();
   }
+  *out = d;
+  return 0;
+}
+
+// TransmissionConstraint::apply_constraint (NADIA/src/TransmissionConstraint.c++:81-172)
+// on buf, or (ill set, no host function) on the transmission function of the
+// exit-surface wave buf holds, written back as the wave
+int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec, const void* ill, int nx, int ny) {
+  int rc;
+  const long long n = (long long)tc->nx * tc->ny;
+  TcDev d{};
+  if ((rc = tc_stats(tc, st, buf, prec, ill, nx, ny, &d))) return rc;
   // with ill the pass also does the T -> ESW round trip the reference does
   // even when nothing changes T
   if (tc->nreg > 0 || tc->charge_flip || tc->unity || ill) {

@@ -10,6 +10,7 @@ enum RowMode {
   ROW_UPDATE = 1,  // inverse row DFT -> P_F x -> algorithm update of x -> forward row DFT
   ROW_WF2 = 2,     // white field: inverse row DFT -> support -> forward row DFT
   ROW_WF4 = 3,     // white field: forward row DFT -> modulus -> store x -> inverse row DFT
+  ROW_TC = 4,      // transmission constraint on src (stored back), then dst = forward row DFT
 };
 
 // Column pass: transforms along x (stride ny, length nx) of every column y.
@@ -39,6 +40,22 @@ struct JobDesc {
   double wf_sx, wf_sy, wf_z12, wf_z23, wf_k, wf_norm;    // white field
 };
 
+// TransmissionConstraint state on the device (NADIA/include/TransmissionConstraint.h)
+struct TcDev {
+  const unsigned char* map;  // region number 1..nregions per pixel, 0 outside
+  int nregions;
+  const double* alpha1;      // per region
+  const double* alpha2;
+  double* c_mean;            // per region; recomputed unless fixed
+  const int* fixed;
+  int charge_flip, flip_sign, unity;
+  // FresnelCDI: the buffer holds the exit-surface wave; the constraint acts
+  // on T = (ESW + L)/L (L = ill, the illumination at the sample) and writes
+  // ESW = (T - 1) L back in the columns j < nx (set_transmission_function)
+  const void* ill;
+  int nx, ny;
+};
+
 struct RowArgs {
   int mode;
   const JobDesc* jobs;  // fused modes: per-job data (blockIdx.y = job)
@@ -55,6 +72,8 @@ struct RowArgs {
   int two;       // the jobs carry a second field (P_F P_S x) after the first
   double post;   // factor applied after the inverse row transform
   int fin_n;     // number of column-pass blocks per field (error finalisation)
+  // ROW_TC
+  TcDev tc;
 };
 
 struct ColArgs {
@@ -88,21 +107,6 @@ template <typename T> void launch_col(const ColArgs& a, int njobs, int v, hipStr
 template <typename T> int col_tile(int nx, int v);
 template <typename T> int row_blocks(int nx, int ny, int v);
 
-// TransmissionConstraint state on the device (NADIA/include/TransmissionConstraint.h)
-struct TcDev {
-  const unsigned char* map;  // region number 1..nregions per pixel, 0 outside
-  int nregions;
-  const double* alpha1;      // per region
-  const double* alpha2;
-  double* c_mean;            // per region; recomputed unless fixed
-  const int* fixed;
-  int charge_flip, flip_sign, unity;
-  // FresnelCDI: the buffer holds the exit-surface wave; the constraint acts
-  // on T = (ESW + L)/L (L = ill, the illumination at the sample) and writes
-  // ESW = (T - 1) L back in the columns j < nx (set_transmission_function)
-  const void* ill;
-  int nx, ny;
-};
 
 // PhaseDiverseCDI frame geometry (NADIA/src/PhaseDiverseCDI.c++): local
 // pixel (i_, j_) of a frame at (x, y) is global pixel

@@ -203,3 +203,16 @@ def test_fresnel_empty_constraint_round_trip(nd, oracle, synth):
     eo = o.iterate_n(5)
     assert (np.abs(eg - eo) / eo).max() <= 1e-10
     assert rel(est.get(), xo) <= 1e-10
+
+
+def test_fresnel_custom_constraint_fused_matches_plain(nd, oracle, synth):
+    """A host constraint function (it needs the T array) on the fused ER path
+    agrees with the plain path."""
+    outs = []
+    for path in (0, 1):
+        est, g, xo, o, tg, tcs = fresnel_constrained(nd, oracle, synth, path=path)
+        tg.set_custom_constraint(lambda a: np.multiply(a, 0.999, out=a))
+        e = g.iterate(4)
+        outs.append((est.get(), e))
+    assert rel(outs[0][0], outs[1][0]) <= 1e-10
+    assert np.abs(outs[0][1] - outs[1][1]).max() <= 1e-10 * np.abs(outs[1][1]).max()
