@@ -280,7 +280,8 @@ int nadia_recon_bytes_estimate(int nx, int ny, int precision, int kind, int n_be
  * precision, kind and size share every launch of the fused passes, so small
  * jobs fill the GPU and the phases of different jobs overlap.  Each job keeps
  * its own state, algorithm and error ring; its results are bit-identical to
- * iterating it alone (same kernels, same per-job reductions).  The
+ * iterating it alone (same kernels, same per-job reductions; a job with a
+ * transmission constraint takes its own launches, as alone).  The
  * reconstructions stay owned by the caller; all must live on the batch's
  * device. */
 int nadia_batch_create(nadia_batch** out);
@@ -301,8 +302,9 @@ int nadia_batch_iterate_each(nadia_batch* b, const int* counts);
  * runs as replays of one captured hipGraph holding `steps` iterations of
  * every launch group (in launch order); the graph is rebuilt only when the
  * batch's launches change.  0 (default) launches eagerly.
- * Results are bit-identical to eager launches.  Applies to batches whose
- * jobs all take the fused path and keep no n-best snapshots. */
+ * Results are bit-identical to eager launches, n-best bookkeeping included.
+ * Applies when every job shares the fused launches (a job on the plain path
+ * or with a transmission constraint makes the batch launch eagerly). */
 int nadia_batch_set_graph(nadia_batch* b, int steps);
 /* graph replays run so far (for tests and measurement) */
 int nadia_batch_graph_replays(const nadia_batch* b, long long* n);
