@@ -19,8 +19,12 @@
  * (delta <-> constant, plane wave <-> delta, Parseval), fftshift identity,
  * unitarity / round trips of every propagator, the named-algorithm coefficient
  * table, fixed points of every algorithm, HIO's inside/outside rule, scipy's
- * 2-D convolution for the shrinkwrap window, and the closed-form Fresnel and
- * white-field phases.
+ * 2-D convolution for the shrinkwrap window, the closed-form Fresnel and
+ * white-field phases, the homogeneous-material fixed point and the alpha = 1
+ * limits of the transmission constraint, PhaseDiverseCDI's geometry and its
+ * one-frame and two-identical-frame limits, a brute-force weighted
+ * correlation for align, the bilinear closed form of the up-sampling, and the
+ * recovery of known frame offsets by adjust_positions.
  */
 #include <math.h>
 #include <stdlib.h>
