@@ -94,6 +94,23 @@ def test_interpolate_closed_form(oracle):
         assert not big[:int(1.5 * s)].any() and not big[:, :int(1.5 * s)].any()
 
 
+def test_interpolate_reproduces_a_ramp(oracle):
+    """Bilinear up-sampling by s is exact on a linear function and centre
+    aligned: with source pixel (i, j) holding 0.3 i - 0.7 j + 20, output
+    pixel (K, L) sits at source coordinates ((K + 0.5)/s - 0.5, (L + 0.5)/s -
+    0.5) and must read the function there; the interior source pixels (all
+    but the outer ring) fill exactly (nx - 2) s x (ny - 2) s output pixels."""
+    nx, ny = 7, 9
+    ii, jj = np.mgrid[0:nx, 0:ny].astype(float)
+    a = 0.3 * ii - 0.7 * jj + 20.0
+    for s in (2, 3, 4):
+        big = oracle.oracle_interpolate(a, s)
+        K, L = np.nonzero(big)
+        assert len(K) == (nx - 2) * s * (ny - 2) * s, s
+        u, v = (K + 0.5) / s - 0.5, (L + 0.5) / s - 0.5
+        assert np.abs(big[K, L] - (0.3 * u - 0.7 * v + 20.0)).max() <= 1e-12, s
+
+
 def make_pd(oracle, d, starts, alg, parallel=False):
     xs, objs = [], []
     for f, inten, sup in zip(d["frames"], d["intensities"], d["supports"]):
