@@ -32,7 +32,8 @@ __device__ __forceinline__ c2_t<T> trans_of(c2_t<T> e, c2_t<T> l) {
   return c2_t<T>{(T)(rn / den), (T)(in / den)};
 }
 
-// get_mag in T, set_mag / set_phase through double temporaries.
+// Complex_2D accessors (NADIA/include/Complex_2D.h:205-290): get_mag in T,
+// set_mag / set_phase through double temporaries.
 template <typename T>
 __device__ __forceinline__ T tc_mag(c2_t<T> v) { return sqrt(v.x * v.x + v.y * v.y); }
 
