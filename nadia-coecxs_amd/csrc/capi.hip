@@ -2285,6 +2285,44 @@ int pd_add(nadia_pd* d, int n) {
   return 0;
 }
 
+// get_result then add_to_object of every frame, in frame order.  At scale 1
+// with integral positions, no frame constraint and every frame inside the
+// object, one pass over the object takes kPdSetMax frames at a time (each
+// object pixel adds its frames in order, as the frame-by-frame launches do);
+// otherwise frame by frame.
+int pd_add_all(nadia_pd* d) {
+  int rc;
+  const int nf = (int)d->frames.size();
+  bool one_pass = d->scale == 1;
+  for (const PdFrame& f : d->frames) {
+    const int lnx = f.r->nx, lny = f.r->ny;
+    if (f.r->tc || f.x != std::floor(f.x) || f.y != std::floor(f.y) || pd_gx(d, 0, f.x) < 0 ||
+        pd_gy(d, 0, f.y) < 0 || pd_gx(d, lnx - 1, f.x) >= d->nx || pd_gy(d, lny - 1, f.y) >= d->ny)
+      one_pass = false;
+  }
+  if (!one_pass) {
+    for (int n = 0; n < nf; n++)
+      if ((rc = pd_add(d, n))) return rc;
+    return 0;
+  }
+  if ((rc = pd_weights(d))) return rc;
+  const PdArgs a = pd_args(d, -1);
+  for (int k0 = 0; k0 < nf; k0 += kPdSetMax) {
+    PdSetList l{};
+    l.n = std::min(kPdSetMax, nf - k0);
+    for (int k = 0; k < l.n; k++) {
+      const PdFrame& f = d->frames[k0 + k];
+      l.f[k] = PdFrameDev{f.weight, f.r->nx, f.r->ny, f.x, f.y};
+      l.small[k] = f.result;
+      l.est[k] = f.r->est->d;
+      l.ill[k] = f.r->kind == NADIA_FRESNEL ? f.r->Ls : nullptr;
+    }
+    DISPATCH(d->prec, k_pd_get_add_all<T>(a, l, dev_ctx(d->dev)->stream));
+    LAUNCHCHK();
+  }
+  return 0;
+}
+
 // update_from_object then set_result (PhaseDiverseCDI.c++:1066-1130,
 // 317-335) of frames first..first+count-1, kPdSetMax frames per launch
 int pd_update_range(nadia_pd* d, int first, int count) {
@@ -2701,8 +2739,7 @@ int nadia_pd_iterate(nadia_pd* d, double* frame_errors) {
       if ((rc = nadia_batch_iterate(d->batch, d->ipc, nullptr))) return rc;
     }
     if ((rc = pd_scale(d, 1 - d->beta))) return rc;
-    for (int n = 0; n < nf; n++)
-      if ((rc = pd_add(d, n))) return rc;
+    if ((rc = pd_add_all(d))) return rc;
   } else {
     for (int n = 0; n < nf; n++) {
       if ((rc = pd_update(d, n))) return rc;

@@ -177,6 +177,7 @@ template <typename T> void k_pd_add(const PdArgs& a, hipStream_t st);
 template <typename T> void k_pd_update(const PdArgs& a, hipStream_t st);
 template <typename T> void k_pd_update_set(const PdArgs& a, const PdSetList& l, hipStream_t st);
 template <typename T> void k_pd_get_add(const PdArgs& a, const void* est, const void* ill, hipStream_t st);
+template <typename T> void k_pd_get_add_all(const PdArgs& a, const PdSetList& l, hipStream_t st);
 template <typename T> void k_pd_cover(const PdArgs& a, const PdFrameDev* frames, int nframes, unsigned char* covered,
                                       hipStream_t st);
 template <typename T> void k_pd_scale(const PdArgs& a, const PdFrameDev* frames, int nframes,

@@ -95,10 +95,12 @@ def test_fp32_frames(nd, oracle, synth):
     assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-5
 
 
-def test_many_frames(nd, oracle, synth):
-    """More frames than one update launch and one error gather take (16):
-    18 planar frames on a 6 x 3 grid of positions, parallel mode."""
-    pos = [(9.0 * (k % 6) - 20, 7.5 * (k // 6) - 8) for k in range(18)]
+@pytest.mark.parametrize("step_y", [7.5, 7.0], ids=["fractional", "integral"])
+def test_many_frames(nd, oracle, synth, step_y):
+    """More frames than one update launch, one object pass and one error
+    gather take (16): 18 planar frames on a 6 x 3 grid of positions, parallel
+    mode; integral positions take the one-pass add of all frames."""
+    pos = [(9.0 * (k % 6) - 20, step_y * (k // 6) - 8) for k in range(18)]
     gpu, cpu = planar_frames(nd, oracle, synth, pos, n=32, alg="HIO")
     pg = nd.PhaseDiverseCDI(beta=0.9, gamma=1.0, parallel=True)
     po = oracle.OraclePhaseDiverse(beta=0.9, gamma=1.0, parallel=True)
