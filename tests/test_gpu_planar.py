@@ -369,3 +369,28 @@ def test_largest_sizes(nd, oracle, synth, n):
             assert rel(x.astype(np.complex128), xo) <= 1e-5
             assert abs(e[-1] - eo) <= 1e-5 * eo
         del g, est
+
+
+@pytest.mark.parametrize("double,tol", [(True, 1e-12), (False, 1e-5)], ids=["fp64", "fp32"])
+def test_consistent_state_is_a_fixed_point_2048(nd, synth, double, tol):
+    """At the full 2048^2 size, through the fused passes: an object x with
+    I = |F x|^2 (NADIA's centred unitary transform) and a support covering it
+    is a fixed point of every algorithm, with error ~0 (P_S x = x and
+    P_F x = x make every update x' = x)."""
+    n = 2048
+    d = synth.planar_star(n=n)
+    sup = d["support"]
+    rng = np.random.default_rng(11)
+    x0 = sup * rng.uniform(0.2, 1.0, (n, n)) * np.exp(1j * rng.uniform(-np.pi, np.pi, (n, n)))
+    inten = np.abs(np.fft.fftshift(np.fft.fft2(x0)) / n) ** 2
+    cdt = np.complex128 if double else np.complex64
+    for alg in ("ER", "HIO", "RAAR", "DM"):
+        est = nd.Complex2D(n, n, double=double, data=x0.astype(cdt))
+        g = nd.PlanarCDI(est)
+        g.set_support(sup)
+        g.set_intensity(inten)
+        g.set_algorithm(getattr(nd, alg))
+        e = g.iterate(2)
+        assert rel(est.get().astype(np.complex128), x0) <= tol, alg
+        assert e.max() <= tol * tol * 100, alg
+        del g, est
