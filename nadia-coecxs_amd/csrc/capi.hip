@@ -401,6 +401,8 @@ int tc_apply(nadia_constraint* tc, hipStream_t st, void* buf, int prec, const vo
              int ny = 0);
 int tc_stats(nadia_constraint* tc, hipStream_t st, const void* buf, int prec, const void* ill, int nx, int ny,
              TcDev* out);
+int tc_view(nadia_constraint* tc, hipStream_t st, const void* ill, int nx, int ny, TcDev* out);
+constexpr int kRowStatsRegions = 4;  // the update pass sums up to this many regions itself
 
 // BaseCDI::apply_support (NADIA/src/BaseCDI.c++:130-134) and
 // FresnelCDI::apply_support (NADIA/src/FresnelCDI.c++:203-215): support,
@@ -650,7 +652,8 @@ GroupKey group_key(const nadia_recon* r) { return GroupKey{r->prec, r->kind, r->
 // error reduction.  evs (may be null) receives per-pass timing events.
 template <typename T>
 int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njobs, int iter, int cv, int rv,
-                 nadia_recon* timer, std::vector<Ev>* evs, hipStream_t st = nullptr) {
+                 nadia_recon* timer, std::vector<Ev>* evs, hipStream_t st = nullptr, const TcDev* tcs = nullptr,
+                 double* tc_part = nullptr) {
   void* twy = twiddles(c, g.ny, g.prec, true);
   void* twx = twiddles(c, g.nx, g.prec, false);
   if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
@@ -692,6 +695,11 @@ int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njob
   if (g.kind != NADIA_FRESNEL_WF) {
     ra.fin_n = g.ny / col_tile<T>(g.nx, cv);
     if ((rc = col(g.kind == NADIA_FRESNEL ? COL_FRESNEL : COL_PLANAR))) return rc;
+    if (tcs) {  // ER with a constraint: the update pass also sums its regions
+      ra.tc = *tcs;
+      ra.tc_part = tc_part;
+      return row(ROW_UPDTC);
+    }
     return row(ROW_UPDATE);
   }
   if ((rc = col(COL_WF1)) || (rc = row(ROW_WF2)) || (rc = col(COL_WF3)) || (rc = row(ROW_WF4))) return rc;
@@ -867,6 +875,32 @@ int iterate_fused(nadia_recon* r, int n) {
 template <typename T>
 int fused_tc_step(nadia_recon* r, int k) {
   int rc;
+  nadia_constraint* tc = r->tc;
+  const bool fres = r->kind == NADIA_FRESNEL;
+  const int rv = r->row_v < 0 ? 0 : r->row_v;
+  if (!tc->custom && tc->nreg <= kRowStatsRegions) {
+    // column pass; update pass storing x and summing the regions; region
+    // means; one pass applying the constraint and rebuilding W
+    TcDev d{};
+    if ((rc = tc_view(tc, S_(r), fres ? r->Ls : nullptr, r->nx, r->ny, &d))) return rc;
+    if ((rc = launch_fused<T>(r->ctx, group_key(r), r->djob, 1, k, r->col_v, rv, r, nullptr, nullptr, &d,
+                              tc->partials)))
+      return rc;
+    if (tc->nreg > 0) k_tc_final(d, tc->partials, row_blocks<T>(r->nx, r->ny, rv), S_(r));
+    void* twy = twiddles(r->ctx, r->ny, r->prec, true);
+    if (!twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
+    RowArgs ra = w_rebuild_args(r, twy);
+    ra.mode = ROW_TC;
+    ra.src = r->est->d;
+    ra.dst = r->W;
+    ra.tc = d;
+    launch_row<T>(ra, 1, rv, S_(r));
+    LAUNCHCHK();
+    r->w_valid = true;
+    r->w_version = r->est->version;
+    if (wants_n_best(r) && (rc = update_n_best(r, r->errs + k, S_(r)))) return rc;
+    return 0;
+  }
   if ((rc = launch_fused<T>(r->ctx, group_key(r), r->djob, 1, k, r->col_v, r->row_v, r, nullptr))) return rc;
   if (r->tc->custom) {
     if ((rc = op_constraint(r, r->est->d))) return rc;
@@ -1873,6 +1907,22 @@ TcDev tc_dev(const nadia_constraint* tc) {
   d.flip_sign = tc->flip_sign;
   d.unity = tc->unity;
   return d;
+}
+
+// The device view of a constraint (uploaded first), acting on T of the exit
+// wave when ill is set.
+int tc_view(nadia_constraint* tc, hipStream_t st, const void* ill, int nx, int ny, TcDev* out) {
+  int rc;
+  if ((rc = tc_upload(tc, st))) return rc;
+  TcDev d = tc_dev(tc);
+  if (ill) {
+    if (tc->custom) return fail(NADIA_EINVAL, "a host constraint function needs the transmission function array");
+    d.ill = ill;
+    d.nx = nx;
+    d.ny = ny;
+  }
+  *out = d;
+  return 0;
 }
 
 // The region means of apply_constraint (TransmissionConstraint.c++:81-120)

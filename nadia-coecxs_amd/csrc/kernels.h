@@ -11,6 +11,7 @@ enum RowMode {
   ROW_WF2 = 2,     // white field: inverse row DFT -> support -> forward row DFT
   ROW_WF4 = 3,     // white field: forward row DFT -> modulus -> store x -> inverse row DFT
   ROW_TC = 4,      // transmission constraint on src (stored back), then dst = forward row DFT
+  ROW_UPDTC = 5,   // ER update storing x = support(P_F x) and the constraint's region sums of x
 };
 
 // Column pass: transforms along x (stride ny, length nx) of every column y.
@@ -72,8 +73,9 @@ struct RowArgs {
   int two;       // the jobs carry a second field (P_F P_S x) after the first
   double post;   // factor applied after the inverse row transform
   int fin_n;     // number of column-pass blocks per field (error finalisation)
-  // ROW_TC
+  // ROW_TC, ROW_UPDTC
   TcDev tc;
+  double* tc_part;  // ROW_UPDTC: per-workgroup region sums, [2*(block*nregions + k)]
 };
 
 struct ColArgs {

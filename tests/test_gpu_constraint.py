@@ -216,3 +216,35 @@ def test_fresnel_custom_constraint_fused_matches_plain(nd, oracle, synth):
         outs.append((est.get(), e))
     assert rel(outs[0][0], outs[1][0]) <= 1e-10
     assert np.abs(outs[0][1] - outs[1][1]).max() <= 1e-10 * np.abs(outs[1][1]).max()
+
+
+@pytest.mark.parametrize("nreg", [4, 5], ids=["4regions", "5regions"])
+def test_planar_er_many_regions(nd, oracle, synth, nreg):
+    """ER with 4 regions (summed inside the update pass) and with 5 (a
+    separate reduction pass) against the oracle."""
+    n = 128
+    d = synth.planar_block(n=n, block=48, support=56)
+    est = nd.Complex2D(n, n, data=d["x0"])
+    g = nd.PlanarCDI(est)
+    xo = d["x0"].astype(np.complex128).copy()
+    o = oracle.OracleCDI(xo, oracle.PLANAR)
+    s0 = n // 2 - 28
+    tg = nd.TransmissionConstraint(n, n)
+    to = oracle.OracleConstraint(n, n)
+    for k in range(nreg):
+        reg = np.zeros((n, n))
+        reg[s0 + 2 + 10 * k:s0 + 12 + 10 * k, s0 + 2:s0 + 54] = 1
+        for tc in (tg, to):
+            tc.add_complex_constraint(reg, 0.3 + 0.1 * k, 0.6 - 0.1 * k)
+    g.set_complex_constraint(tg)
+    o.set_complex_constraint(to)
+    for obj in (g, o):
+        obj.set_support(d["support"])
+        obj.set_intensity(d["intensity"])
+        obj.set_algorithm(oracle.ER)
+    eg = g.iterate(8)
+    eo = o.iterate_n(8)
+    assert (np.abs(eg - eo) / eo).max() <= 1e-10
+    assert rel(est.get(), xo) <= 1e-10
+    for k in range(nreg):
+        assert abs(tg.c_mean(k) - to.c_mean(k)) <= 1e-10 * abs(to.c_mean(k))
