@@ -176,6 +176,7 @@ struct nadia_recon {
   // Fresnel
   void *L, *Ls;
   double norm, wl, zfd, zfs, pixel, phase_factor;
+  void *phx, *phy;  // per-axis factors of the Fresnel quadratic phase (complex T)
   bool fresnel_ready;
   // white field
   double z12, z23, wf_sx, wf_sy, wf_k, wf_norm;
@@ -622,6 +623,8 @@ JobDesc make_jobdesc(const nadia_recon* r, void* X, void* W, double* errs) {
   // when a PF or PSF term exists, else P_F P_S x
   j.err_field = (r->algorithm == NADIA_ER || r->alg[A_PF] != 0 || r->alg[A_PSF] != 0) ? 0 : 1;
   j.phase_factor = r->phase_factor;
+  j.fx = r->phx;
+  j.fy = r->phy;
   j.wf_sx = r->wf_sx;
   j.wf_sy = r->wf_sy;
   j.wf_z12 = r->z12;
@@ -1301,7 +1304,7 @@ int nadia_recon_destroy(nadia_recon* r) {
   (void)hipStreamSynchronize(S_(r));
   void* ptrs[] = {r->S, r->Q, r->beam, r->W, r->t[0], r->t[1], r->t[2], r->t[3], r->conv, r->rtmp,
                   r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs, r->ttmp,
-                  r->dbest_err, r->dbest_perm, r->dbest_bufs, r->dbest_target};
+                  r->dbest_err, r->dbest_perm, r->dbest_bufs, r->dbest_target, r->phx, r->phy};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (void* p : r->best) (void)hipFree(p);
@@ -1695,6 +1698,31 @@ int nadia_fresnel_set_experimental_parameters(nadia_recon* r, double wl, double 
   r->phase_factor = pixel * pixel * M_PI / (wl) * ((1 / zfd) - (1 / zsd));
   r->fresnel_ready = true;
   int rc;
+  // the quadratic phase about ((nx-1)/2, (ny-1)/2) is separable:
+  // exp(i f (dx^2 + dy^2)) = exp(i f dx^2) exp(i f dy^2); the fused column
+  // pass multiplies the two per-axis factors instead of evaluating cos/sin
+  // per pixel
+  if (!r->phx && (rc = alloc(&r->phx, (size_t)r->nx * r->esz))) return rc;
+  if (!r->phy && (rc = alloc(&r->phy, (size_t)r->ny * r->esz))) return rc;
+  for (int axis = 0; axis < 2; axis++) {
+    const int n = axis ? r->ny : r->nx;
+    std::vector<double> h(2 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+      const double dd = (n - 1) / 2.0 - i;
+      const double ph = r->phase_factor * (dd * dd);
+      h[2 * i] = std::cos(ph);
+      h[2 * i + 1] = std::sin(ph);
+    }
+    void* dst = axis ? r->phy : r->phx;
+    if (r->prec == NADIA_FP64) {
+      HIPCHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, S_(r)));
+      HIPCHK(hipStreamSynchronize(S_(r)));
+    } else {
+      std::vector<float> f(h.begin(), h.end());
+      HIPCHK(hipMemcpyAsync(dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, S_(r)));
+      HIPCHK(hipStreamSynchronize(S_(r)));
+    }
+  }
   if (!r->Ls && (rc = alloc(&r->Ls, r->npix * r->esz))) return rc;
   HIPCHK(hipMemcpyAsync(r->Ls, r->L, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
   if ((rc = op_from_detector(r, r->Ls))) return rc;

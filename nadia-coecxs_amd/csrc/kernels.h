@@ -38,6 +38,8 @@ struct JobDesc {
   int er;             // error-reduction special case
   int err_field;      // field whose projection error get_error() reports
   double phase_factor;                                   // Fresnel
+  const void* fx;     // Fresnel: (cos, sin) of phase_factor (xmid - x)^2 per row x, complex T
+  const void* fy;     // the same per column y
   double wf_sx, wf_sy, wf_z12, wf_z23, wf_k, wf_norm;    // white field
 };
 
