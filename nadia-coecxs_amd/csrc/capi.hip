@@ -114,18 +114,35 @@ void* twiddles(DeviceCtx* c, int n, int prec, bool row) {
   void* dptr = nullptr;
   size_t esz = prec == NADIA_FP64 ? 16 : 8;
   if (hipMalloc(&dptr, (size_t)(total > 0 ? total : 1) * esz) != hipSuccess) return nullptr;
+  hipError_t e = hipSuccess;
   if (prec == NADIA_FP64) {
     std::vector<double> h(2 * (size_t)total);
     for (int i = 0; i < total; ++i) { h[2 * i] = re[i]; h[2 * i + 1] = im[i]; }
-    if (total) (void)hipMemcpy(dptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (total) e = hipMemcpy(dptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
   } else {
     std::vector<float> h(2 * (size_t)total);
     for (int i = 0; i < total; ++i) { h[2 * i] = (float)re[i]; h[2 * i + 1] = (float)im[i]; }
-    if (total) (void)hipMemcpy(dptr, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (total) e = hipMemcpy(dptr, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {  // an unfilled table is never cached
+    (void)hipFree(dptr);
+    return nullptr;
   }
   std::lock_guard<std::mutex> lk(g_mu);
-  c->twiddles[key] = dptr;
+  auto ins = c->twiddles.emplace(key, dptr);
+  if (!ins.second) {  // another thread built it meanwhile
+    (void)hipFree(dptr);
+    return ins.first->second;
+  }
   return dptr;
+}
+
+// Both tables of an nx x ny transform, built before any stream capture (the
+// allocation and the synchronous upload are not allowed inside one).
+int ensure_twiddles(DeviceCtx* c, int nx, int ny, int prec) {
+  if (!twiddles(c, ny, prec, true) || !twiddles(c, nx, prec, false))
+    return fail(NADIA_EHIP, "twiddle table allocation failed");
+  return 0;
 }
 
 enum { A_PSF = 0, A_PFS = 1, A_PS = 2, A_PF = 3, A_PI = 4 };
@@ -256,11 +273,14 @@ struct nadia_batch {
   bool hev_pending;
   // graph mode: uniform sweeps run as replays of a captured graph of
   // `gsteps` iterations; gsig describes the launches it was captured for
+  // (membership generation, groups, jobs and their n-best state: the graph
+  // holds each job's raw n-best and error-ring pointers)
   int graph_steps;
   hipGraphExec_t gexec;
   int gsteps;
   std::vector<long long> gsig;
   long long graph_replays;
+  unsigned long long gen;  // bumped by every add / remove
 };
 
 namespace {
@@ -1696,7 +1716,9 @@ int nadia_fresnel_set_experimental_parameters(nadia_recon* r, double wl, double 
   r->zfs = zfs;
   const double zsd = zfd - zfs;
   r->phase_factor = pixel * pixel * M_PI / (wl) * ((1 / zfd) - (1 / zsd));
-  r->fresnel_ready = true;
+  // ready only once the factor tables and the illumination at the sample
+  // exist (the fused column pass reads the tables through raw pointers)
+  r->fresnel_ready = false;
   int rc;
   // the quadratic phase about ((nx-1)/2, (ny-1)/2) is separable:
   // exp(i f (dx^2 + dy^2)) = exp(i f dx^2) exp(i f dy^2); the fused column
@@ -1725,9 +1747,13 @@ int nadia_fresnel_set_experimental_parameters(nadia_recon* r, double wl, double 
   }
   if (!r->Ls && (rc = alloc(&r->Ls, r->npix * r->esz))) return rc;
   HIPCHK(hipMemcpyAsync(r->Ls, r->L, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
-  if ((rc = op_from_detector(r, r->Ls))) return rc;
-  HIPCHK(hipStreamSynchronize(S_(r)));
-  return 0;
+  r->fresnel_ready = true;  // op_from_detector checks it
+  if ((rc = op_from_detector(r, r->Ls)) == 0) {
+    hipError_t e = hipStreamSynchronize(S_(r));
+    if (e != hipSuccess) rc = fail(NADIA_EHIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+  }
+  if (rc) r->fresnel_ready = false;
+  return rc;
 }
 
 int nadia_fresnel_setup(nadia_recon* r, const void* white_field, double wl, double zfd, double zfs, double pixel,
@@ -2979,6 +3005,7 @@ int nadia_batch_add(nadia_batch* b, nadia_recon* r) {
   for (nadia_recon* j : b->jobs)
     if (j == r) return fail(NADIA_EINVAL, "reconstruction already in the batch");
   b->jobs.push_back(r);
+  b->gen++;  // a captured graph never replays for another membership
   return 0;
 }
 
@@ -2987,6 +3014,7 @@ int nadia_batch_remove(nadia_batch* b, nadia_recon* r) {
   for (size_t i = 0; i < b->jobs.size(); i++)
     if (b->jobs[i] == r) {
       b->jobs.erase(b->jobs.begin() + i);
+      b->gen++;
       return 0;
     }
   return fail(NADIA_EINVAL, "reconstruction not in the batch");
@@ -3057,11 +3085,15 @@ int ensure_side(DeviceCtx* c, int n) {
 // is an independent chain of passes (groups touch disjoint jobs), forked off
 // the device stream so the graph holds them as parallel branches.
 int capture_batch_graph(nadia_batch* b, DeviceCtx* c, const std::vector<BatchLaunch>& launches, int steps) {
-  std::vector<long long> sig{steps, (long long)(intptr_t)b->djobs};
+  std::vector<long long> sig{steps, (long long)(intptr_t)b->djobs, (long long)b->gen};
   for (const BatchLaunch& L : launches) {
     long long v[] = {L.g.prec, L.g.kind, L.g.nx, L.g.ny, L.g.two, (long long)(L.d - b->djobs),
                      (long long)L.rem.size(), L.cv, L.rv};
     sig.insert(sig.end(), v, v + 9);
+    for (const nadia_recon* r : L.jobs) {
+      sig.push_back((long long)(intptr_t)r);
+      sig.push_back(wants_n_best(r) ? r->n_best : 0);
+    }
   }
   if (b->gexec && sig == b->gsig) return 0;
   if (b->gexec) {
@@ -3195,6 +3227,8 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
         if (rc) return rc;
       }
       nadia_recon* r0 = b->jobs[kv.second[0]];
+      // both twiddle tables exist before a graph capture can start
+      if ((rc = ensure_twiddles(c, kv.first.nx, kv.first.ny, kv.first.prec))) return rc;
       BatchLaunch L{kv.first, b->djobs + host.size(), r0->col_v, r0->row_v, {}, {}};
       L.rebuild_w = all_stale;
       for (int j : kv.second) {

@@ -222,7 +222,10 @@ class DeviceBackend:
         self.inputs = inputs
         self.double = double
         self.batch = nd.Batch()
-        self.idle = collections.defaultdict(list)   # n -> [(est, recon)]
+        # (n, has beam stop) -> [(est, recon)]: a reconstruction keeps its
+        # beam stop once set (BaseCDI has no way to clear it), so it is only
+        # reused by requests with the same beam-stop state
+        self.idle = collections.defaultdict(list)
         self.idle_bytes = 0
         self._bytes = {}
 
@@ -236,11 +239,11 @@ class DeviceBackend:
 
     def release_idle(self, nbytes):
         freed = 0
-        for n in sorted(self.idle, reverse=True):
-            while self.idle[n] and freed < nbytes:
-                est, p = self.idle[n].pop()
+        for key in sorted(self.idle, reverse=True):
+            while self.idle[key] and freed < nbytes:
+                est, p = self.idle[key].pop()
                 del p, est
-                freed += self._bytes[n]
+                freed += self._bytes[key[0]]
         self.idle_bytes -= freed
         return freed
 
@@ -248,23 +251,29 @@ class DeviceBackend:
         nd = self.nd
         d = self.inputs(req)
         cdt = np.complex128 if self.double else np.complex64
-        if self.idle[req.n]:
-            est, p = self.idle[req.n].pop()
+        key = (req.n, "beam_stop" in d)
+        if self.idle[key]:
+            est, p = self.idle[key].pop()
             self.idle_bytes -= self._bytes[req.n]
             est.set(d["x0"].astype(cdt))
+            # back to a new object's algorithm state (ER, beta 0.9,
+            # NADIA/src/BaseCDI.c++:33-66) so the request below selects its
+            # algorithm exactly as on a fresh object
+            p.set_relaxation_parameter(0.9)
+            p.set_algorithm(ER)
         else:
             est = nd.Complex2D(req.n, req.n, double=self.double, data=d["x0"].astype(cdt))
             p = nd.PlanarCDI(est)
         p.set_support(d["support"])
         p.set_intensity(d["intensity"])
-        if "beam_stop" in d:
+        if key[1]:
             p.set_beam_stop(d["beam_stop"])
         # set_relaxation_parameter before set_algorithm: selecting the
         # algorithm a job already has does nothing (NADIA/src/BaseCDI.c++:194-195)
         p.set_relaxation_parameter(req.beta)
         p.set_algorithm(req.algorithm)
         self.batch.add(p)
-        return (req, est, p)
+        return (req, est, p, key[1])
 
     def run(self, handles, counts):
         # the batch holds exactly the active jobs, in admission order
@@ -278,11 +287,11 @@ class DeviceBackend:
         self.nd.synchronize()
 
     def finish(self, handle, keep):
-        req, est, p = handle
+        req, est, p, beam = handle
         err = p.get_error()
         res = est.get() if keep else None
         self.batch.remove(p)
-        self.idle[req.n].append((est, p))
+        self.idle[(req.n, beam)].append((est, p))
         self.idle_bytes += self._bytes[req.n]
         return err, res
 

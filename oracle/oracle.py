@@ -100,6 +100,9 @@ def _lib(double=True):
         "oalign": (I, [P, I, I, P, I, I, P, P, I, I, I, I, P, P, D]),
         "ointerpolate": (None, [P, I, I, P, I, I]),
     }
+    R = D if double else ctypes.c_float  # the oracle's `real`
+    S = ctypes.c_size_t
+    sig.update({"or_add": (None, [P, P, S, R]), "or_scale": (None, [P, S, R]), "or_get_norm": (R, [P, S])})
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
         f.restype = res
@@ -125,6 +128,24 @@ def invert(a, scale=True, double=True):
     lib = _lib(double)
     lib.or_invert(_ptr(a), a.shape[0], a.shape[1], 1 if scale else 0)
     return a
+
+
+def add(a, b, scale=1.0, double=True):
+    """Complex_2D::add (a += scale * b), in place."""
+    _lib(double).or_add(_ptr(a), _ptr(b), a.size, scale)
+    return a
+
+
+def scale(a, factor, double=True):
+    """Complex_2D::scale, in place."""
+    _lib(double).or_scale(_ptr(a), a.size, factor)
+    return a
+
+
+def get_norm(a, double=True):
+    """Complex_2D::get_norm: sqrt of the sum of |c|^2, accumulated in the
+    array's real type."""
+    return _lib(double).or_get_norm(_ptr(a), a.size)
 
 
 class OracleConstraint:

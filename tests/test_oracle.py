@@ -454,7 +454,7 @@ def test_wf_propagators_closed_form_and_round_trip(oracle):
     assert rel(b, a) < 1e-12
 
 
-def test_wf_support_disk_and_iteration_fixed_point(oracle):
+def test_wf_support_softened_disk(oracle):
     n = 32
     x = np.zeros((n, n), complex)
     o = oracle.OracleCDI(x, oracle.FRESNEL_WF)
@@ -471,9 +471,9 @@ def test_wf_support_disk_and_iteration_fixed_point(oracle):
     k = np.arange(-5, 6)
     g = np.exp(-(k[:, None] ** 2 + k[None, :] ** 2) / 18.0)
     assert rel(s, convolve2d(disk, g, mode="same")) < 1e-14
-    # a field whose zone-plate-plane image equals itself times the support and
-    # whose modulus is the measured one is a fixed point
     assert s.max() > 1.0  # left unnormalised like the reference
+    # (the white-field iteration itself, and its fixed point, are pinned in
+    # test_oracle_pinning.py)
 
 
 # ------------------------------------------------ TransmissionConstraint ----
