@@ -102,6 +102,15 @@ int nadia_field_add(nadia_field* f, const nadia_field* other, double scale);
 int nadia_field_norm(const nadia_field* f, double* out);
 /* get_2d(MAG, result) (src/Complex_2D.c++:158-165) into a host real array */
 int nadia_field_magnitude(const nadia_field* f, void* host_real);
+/* No counterpart in the reference: nadia_field_magnitude for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The magnitudes are
+ * staged in a fixed device array of the library (32 MiB per GPU, held from
+ * library load to process end) instead of a per-call allocation; fields
+ * needing more (more than 2048 x 2048 FP64 or 4096 x 2048 FP32 pixels) are
+ * refused with NADIA_ESIZE.  Calls that overlap on one GPU race on the
+ * array. */
+int nadia_field_magnitude_noalloc(const nadia_field* f, void* host_real);
 
 /* ---- BaseCDI (NADIA/include/BaseCDI.h:66-522, src/BaseCDI.c++) ---- */
 

@@ -66,6 +66,7 @@ def _sig(lib):
         "nadia_field_add": (I, [P, P, D]),
         "nadia_field_norm": (I, [P, DP]),
         "nadia_field_magnitude": (I, [P, P]),
+        "nadia_field_magnitude_noalloc": (I, [P, P]),
         "nadia_recon_create": (I, [P, I, I, PP]),
         "nadia_recon_destroy": (I, [P]),
         "nadia_recon_set_support": (I, [P, P, I]),

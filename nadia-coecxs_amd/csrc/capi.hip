@@ -59,6 +59,9 @@ int fail(int code, const char* fmt, ...) {
 
 struct DeviceCtx {
   hipStream_t stream = nullptr;
+  // this device's address of the file-scope `scratch` array (set once, by
+  // the first allocation-free call on the device)
+  void* scratch = nullptr;
   std::map<std::pair<int, int>, void*> twiddles;  // (n, precision) -> table
   // side streams (default priority) and events that fork independent groups
   // of a batch off `stream` while a graph is captured
@@ -147,8 +150,48 @@ int ensure_twiddles(DeviceCtx* c, int nx, int ny, int prec) {
 
 enum { A_PSF = 0, A_PFS = 1, A_PS = 2, A_PF = 3, A_PI = 4 };
 
+// Size of the device array behind nadia_field_magnitude_noalloc: the
+// magnitudes of a 2048 x 2048 FP64 (or 4096 x 2048 FP32) field.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+
 int alloc(void** p, size_t bytes);
 int env_int(const char* name, int dflt);
+
+}  // namespace
+
+// Staging of nadia_field_magnitude_noalloc: part of the library's code
+// object, so every GPU the library runs on holds its own copy from module
+// load to process end and no call allocates it.  In host code this name is
+// a host-side stand-in: kernels and copies use the per-device address from
+// hipGetSymbolAddress (DeviceCtx::scratch).
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// This device's address of `scratch`, looked up once per device with that
+// device current.
+int noalloc_scratch(int dev, void** out) {
+  DeviceCtx* c = dev_ctx(dev);
+  if (!c) return fail(NADIA_EHIP, "stream creation failed");
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (c->scratch) {
+      *out = c->scratch;
+      return 0;
+    }
+  }
+  int cur = 0;
+  HIPCHK(hipGetDevice(&cur));
+  if (cur != dev) HIPCHK(hipSetDevice(dev));
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur != dev) (void)hipSetDevice(cur);
+  if (e != hipSuccess || !p) return fail(NADIA_EHIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  std::lock_guard<std::mutex> lk(g_mu);
+  c->scratch = p;
+  *out = p;
+  return 0;
+}
 
 }  // namespace
 
@@ -1196,6 +1239,30 @@ int nadia_field_magnitude(const nadia_field* f, void* host_real) {
   HIPCHK(hipMemcpyAsync(host_real, tmp, (size_t)f->nx * f->ny * rsz, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   (void)hipFree(tmp);
+  return 0;
+}
+
+// nadia_field_magnitude staged through the library's fixed device array
+// instead of an allocation: no hipMalloc / hipFree.  Calls that overlap on
+// one GPU share the array (race); every element it stages is written before
+// it is read, so nothing needs clearing.
+int nadia_field_magnitude_noalloc(const nadia_field* f, void* host_real) {
+  if (!f || !host_real) return fail(NADIA_EINVAL, "null argument");
+  const long long n = (long long)f->nx * f->ny;
+  const size_t bytes = (size_t)n * (f->esz / 2);
+  if (bytes > kNoAllocScratchBytes)
+    return fail(NADIA_ESIZE,
+                "nadia_field_magnitude_noalloc: a %dx%d field needs %zu bytes of staging, more than the %zu it holds "
+                "(use nadia_field_magnitude)",
+                f->nx, f->ny, bytes, kNoAllocScratchBytes);
+  void* p = nullptr;
+  int rc;
+  if ((rc = noalloc_scratch(f->dev, &p))) return rc;
+  DeviceCtx* c = dev_ctx(f->dev);
+  DISPATCH(f->prec, k_magnitude<T>(f->d, p, n, c->stream));
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(host_real, p, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
 
