@@ -716,10 +716,14 @@ GroupKey group_key(const nadia_recon* r) { return GroupKey{r->prec, r->kind, r->
 // variants cv / rv; on stream st (default: the device's stream).  PlanarCDI and FresnelCDI: column pass (modulus
 // projection) + row pass (update).  FresnelCDI_WF: four passes and a final
 // error reduction.  evs (may be null) receives per-pass timing events.
+// FresnelCDI_WF: the error of an iteration is reduced by the next
+// iteration's first column pass; jobs [fin_first, fin_first + fin_count)
+// (fin_count < 0: all) run no further iteration in this sequence and get a
+// separate reduction after this one.
 template <typename T>
 int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njobs, int iter, int cv, int rv,
                  nadia_recon* timer, std::vector<Ev>* evs, hipStream_t st = nullptr, const TcDev* tcs = nullptr,
-                 double* tc_part = nullptr) {
+                 double* tc_part = nullptr, int fin_first = 0, int fin_count = -1) {
   void* twy = twiddles(c, g.ny, g.prec, true);
   void* twx = twiddles(c, g.nx, g.prec, false);
   if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
@@ -768,9 +772,14 @@ int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njob
     }
     return row(ROW_UPDATE);
   }
+  ca.iter = iter;
+  ca.fin_n = row_blocks<T>(g.nx, g.ny, rv);
   if ((rc = col(COL_WF1)) || (rc = row(ROW_WF2)) || (rc = col(COL_WF3)) || (rc = row(ROW_WF4))) return rc;
-  k_final_jobs(djobs, njobs, row_blocks<T>(g.nx, g.ny, rv), iter, st);
-  LAUNCHCHK();
+  if (fin_count < 0) fin_count = njobs - fin_first;
+  if (fin_count > 0) {
+    k_final_jobs(djobs + fin_first, fin_count, ca.fin_n, iter, st);
+    LAUNCHCHK();
+  }
   return 0;
 }
 
@@ -803,7 +812,9 @@ int time_variant(nadia_recon* r, const JobDesc* dj, int cv, int rv, int reps, fl
   int rc = 0;
   const GroupKey g = group_key(r);
   HIPCHK(hipEventRecord(a, S_(r)));
-  for (int k = 0; k < reps && !rc; ++k) rc = launch_fused<T>(r->ctx, g, dj, 1, k, cv, rv, nullptr, nullptr);
+  for (int k = 0; k < reps && !rc; ++k)
+    rc = launch_fused<T>(r->ctx, g, dj, 1, k, cv, rv, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                         k == reps - 1 ? 1 : 0);
   HIPCHK(hipEventRecord(b, S_(r)));
   HIPCHK(hipEventSynchronize(b));
   HIPCHK(hipEventElapsedTime(ms, a, b));
@@ -921,7 +932,8 @@ int iterate_fused(nadia_recon* r, int n) {
   const GroupKey g = group_key(r);
   const bool nb = wants_n_best(r);
   for (int k = 0; k < n; ++k) {
-    if ((rc = launch_fused<T>(r->ctx, g, r->djob, 1, k, r->col_v, r->row_v, r, r->timing ? &evs : nullptr)))
+    if ((rc = launch_fused<T>(r->ctx, g, r->djob, 1, k, r->col_v, r->row_v, r, r->timing ? &evs : nullptr,
+                              nullptr, nullptr, nullptr, 0, k == n - 1 ? 1 : 0)))
       return rc;
     if (nb && (rc = update_n_best(r, r->errs + k, S_(r)))) return rc;
   }
@@ -1945,6 +1957,33 @@ int nadia_wf_setup(nadia_recon* r, double wl, double z12, double z23, double pix
   r->wf_sy = wl * z23 / (pixel * r->ny);
   r->wf_k = M_PI / wl;
   r->wf_norm = 1 / (std::sqrt((double)(r->nx * r->ny)));
+  r->wf_ready = false;
+  // phi = (pi/lambda)(rho^2/z12 + rho^2/z23), rho^2 = ax^2 + ay^2, is a sum
+  // of per-axis terms: the fused column passes multiply two factor tables
+  // exp(i phi_x) exp(i phi_y) instead of evaluating cos / sin per pixel
+  int rc;
+  if (!r->phx && (rc = alloc(&r->phx, (size_t)r->nx * r->esz))) return rc;
+  if (!r->phy && (rc = alloc(&r->phy, (size_t)r->ny * r->esz))) return rc;
+  for (int axis = 0; axis < 2; axis++) {
+    const int n = axis ? r->ny : r->nx;
+    const double sc = axis ? r->wf_sy : r->wf_sx;
+    std::vector<double> h(2 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+      const double a = sc * ((n - 1) / 2.0 - i);
+      const double ph = (a * a / z12 + a * a / z23) * r->wf_k;
+      h[2 * i] = std::cos(ph);
+      h[2 * i + 1] = std::sin(ph);
+    }
+    void* dst = axis ? r->phy : r->phx;
+    if (r->prec == NADIA_FP64) {
+      HIPCHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, S_(r)));
+      HIPCHK(hipStreamSynchronize(S_(r)));
+    } else {
+      std::vector<float> f(h.begin(), h.end());
+      HIPCHK(hipMemcpyAsync(dst, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice, S_(r)));
+      HIPCHK(hipStreamSynchronize(S_(r)));
+    }
+  }
   r->wf_ready = true;
   return 0;
 }
@@ -3183,7 +3222,8 @@ int capture_batch_graph(nadia_batch* b, DeviceCtx* c, const std::vector<BatchLau
       break;
     }
     for (int k = 0; k < steps && !rc; ++k) {
-      DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, (int)L.rem.size(), k, L.cv, L.rv, nullptr, nullptr, s2));
+      DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, (int)L.rem.size(), k, L.cv, L.rv, nullptr, nullptr, s2,
+                                              nullptr, nullptr, 0, k == steps - 1 ? -1 : 0));
       if (!rc) rc = L.n_best_after((int)L.rem.size(), k, s2);
     }
     if (!rc && i > 0 && hipEventRecord(c->join[i - 1], s2) != hipSuccess)
@@ -3366,7 +3406,10 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       for (int k = eager_from; k < chunk; ++k) {
         const int a = L.active(k);
         if (a == 0) break;  // jobs are sorted by remaining count: none later either
-        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, k - eager_from, L.cv, L.rv, nullptr, nullptr, s2));
+        // the jobs whose last iteration this is: [active(k + 1), a)
+        const int a_next = k + 1 < chunk ? L.active(k + 1) : 0;
+        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, k - eager_from, L.cv, L.rv, nullptr, nullptr, s2,
+                                                nullptr, nullptr, a_next, a - a_next));
         if (rc || (rc = L.n_best_after(a, k - eager_from, s2))) return rc;
       }
     }

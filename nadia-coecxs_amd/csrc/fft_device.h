@@ -153,10 +153,13 @@ __device__ __forceinline__ void dft(c2_t<T> (&a)[R]) {
   }
 }
 
-// LDS position of transform position p: one pad element after every 8, so
-// the stride-8 writes of the first exchanges spread over all banks.
-template <int N> constexpr int lds_len() { return N + N / 8; }
-__device__ __forceinline__ int lpos(int p) { return p + (p >> 3); }
+// LDS position of transform position p: one pad element after every 2^PS
+// positions, so the strided writes of the first exchanges spread over the
+// banks.  PS = 3 suits one element per position (row passes); a column tile
+// stores CB elements per position, and for its 16-point plans one pad per
+// 16 positions leaves no bank conflict in any exchange.
+template <int N, int PS = 3> constexpr int lds_len() { return N + (N >> PS); }
+template <int PS = 3> __device__ __forceinline__ int lpos(int p) { return p + (p >> PS); }
 
 // Twiddles of stage S for the NB = P/R butterflies of one thread:
 // w[q][m-1] = W_{L R}^{k m}, k = (t + q*TPT) mod L (sign -1 values).
@@ -189,7 +192,7 @@ __device__ __forceinline__ void load_tw(StageTw<T, N, P, S>& w, int t, const c2_
 // stage S+1 are loaded before stage S's exchange barrier, so their latency
 // overlaps the exchange.  Every thread of the workgroup must call this (it
 // contains barriers).
-template <typename T, int N, int P, int DIR, int S>
+template <typename T, int N, int P, int DIR, int S, int PS = 3>
 __device__ __forceinline__ void fft_stage(c2_t<T> (&v)[P], c2_t<T>* lds, int lstride, int t,
                                           const c2_t<T>* __restrict__ tw, const StageTw<T, N, P, S>& ws) {
   using C = c2_t<T>;
@@ -222,15 +225,16 @@ __device__ __forceinline__ void fft_stage(c2_t<T> (&v)[P], c2_t<T>* lds, int lst
       } else {
         const int k = j % L;
         const int base = (j - k) * R + k;
-        if constexpr (L % 8 == 0 || (L == 1 && 8 % R == 0)) {
-          // lpos(base + m*L) = lpos(base) + m*L + m*L/8: m*L is a multiple
-          // of 8, or base is a multiple of R and m < R divides 8
-          const int lb = lpos(base);
+        constexpr int PAD = 1 << PS;
+        if constexpr (L % PAD == 0 || (L == 1 && PAD % R == 0)) {
+          // lpos(base + m*L) = lpos(base) + m*L + (m*L >> PS): m*L is a
+          // multiple of 2^PS, or base is a multiple of R and m < R divides 2^PS
+          const int lb = lpos<PS>(base);
 #pragma unroll
-          for (int m = 0; m < R; ++m) lds[(lb + m * L + (m * L) / 8) * lstride] = a[m];
+          for (int m = 0; m < R; ++m) lds[(lb + m * L + ((m * L) >> PS)) * lstride] = a[m];
         } else {
 #pragma unroll
-          for (int m = 0; m < R; ++m) lds[lpos(base + m * L) * lstride] = a[m];
+          for (int m = 0; m < R; ++m) lds[lpos<PS>(base + m * L) * lstride] = a[m];
         }
       }
     }
@@ -238,25 +242,25 @@ __device__ __forceinline__ void fft_stage(c2_t<T> (&v)[P], c2_t<T>* lds, int lst
       StageTw<T, N, P, S + 1> wn;
       load_tw<T, N, P, S + 1>(wn, t, tw);
       __syncthreads();
-      if constexpr (TPT % 8 == 0) {
-        const int lt = lpos(t);
+      if constexpr (TPT % (1 << PS) == 0) {
+        const int lt = lpos<PS>(t);
 #pragma unroll
-        for (int i = 0; i < P; ++i) v[i] = lds[(lt + i * (TPT + TPT / 8)) * lstride];
+        for (int i = 0; i < P; ++i) v[i] = lds[(lt + i * (TPT + (TPT >> PS))) * lstride];
       } else {
 #pragma unroll
-        for (int i = 0; i < P; ++i) v[i] = lds[lpos(t + i * TPT) * lstride];
+        for (int i = 0; i < P; ++i) v[i] = lds[lpos<PS>(t + i * TPT) * lstride];
       }
       __syncthreads();
-      fft_stage<T, N, P, DIR, S + 1>(v, lds, lstride, t, tw, wn);
+      fft_stage<T, N, P, DIR, S + 1, PS>(v, lds, lstride, t, tw, wn);
     }
   }
 }
 
-template <typename T, int N, int P, int DIR>
+template <typename T, int N, int P, int DIR, int PS = 3>
 __device__ __forceinline__ void fft_stages(c2_t<T> (&v)[P], c2_t<T>* lds, int lstride, int t,
                                            const c2_t<T>* __restrict__ tw) {
   StageTw<T, N, P, 0> w0;
-  fft_stage<T, N, P, DIR, 0>(v, lds, lstride, t, tw, w0);
+  fft_stage<T, N, P, DIR, 0, PS>(v, lds, lstride, t, tw, w0);
 }
 
 // Wavefront + workgroup sum of two doubles; result valid in thread 0.
@@ -284,6 +288,21 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
     a = sa;
     b = sb;
   }
+}
+
+// The error of an iteration from n partial (diff, norm) pairs: the first 256
+// threads of the workgroup sum them in a fixed order and the rest add
+// zeros, so the value does not depend on which kernel (and workgroup size)
+// reduces it.  Result in out[0]; `red` as for block_sum2.
+__device__ __forceinline__ void finalize_err256(const double* p, int n, double* out, double* red) {
+  double d = 0, m = 0;
+  if (threadIdx.x < 256)
+    for (int i = threadIdx.x; i < n; i += 256) {
+      d += p[2 * i];
+      m += p[2 * i + 1];
+    }
+  block_sum2(d, m, red);
+  if (threadIdx.x == 0) out[0] = d / m;
 }
 
 __device__ __forceinline__ double block_max(double a, double* red) {

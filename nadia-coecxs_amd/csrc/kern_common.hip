@@ -38,17 +38,12 @@ __global__ void __launch_bounds__(256) final_sum1_kernel(const double* p, int n,
   if (threadIdx.x == 0) out[0] = a;
 }
 
-// one workgroup per job: its row-pass partial sums -> errs[iter]
+// one workgroup per job: its row-pass partial sums -> errs[iter] (the same
+// order as the reduction folded into the next column pass)
 __global__ void __launch_bounds__(256) final_jobs_kernel(const JobDesc* jobs, int n, int iter) {
   __shared__ double red[16];
   const JobDesc* j = jobs + blockIdx.x;
-  double a = 0, b = 0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    a += j->partials[2 * i];
-    b += j->partials[2 * i + 1];
-  }
-  block_sum2(a, b, red);
-  if (threadIdx.x == 0) j->errs[iter] = a / b;
+  finalize_err256(j->partials, n, j->errs + iter, red);
 }
 
 // one workgroup per region: c_mean = sum |log mag| / sum |phase| over the
@@ -70,7 +65,7 @@ void k_tc_final(const TcDev& d, const double* partials, int nblocks, hipStream_t
 }
 
 void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st) {
-  hipLaunchKernelGGL(final_jobs_kernel, dim3(njobs), dim3(256), 0, st, jobs, n, iter);
+  if (njobs > 0) hipLaunchKernelGGL(final_jobs_kernel, dim3(njobs), dim3(256), 0, st, jobs, n, iter);
 }
 
 // BaseCDI::update_n_best (NADIA/src/BaseCDI.c++:390-413) on the device: the

@@ -93,6 +93,9 @@ struct ColArgs {
   // fused modes
   double sf;  // 1/sqrt(nx*ny)
   double xmid, ymid;
+  // COL_WF1: the previous iteration's error (ring slot iter - 1, from fin_n
+  // row-pass partial pairs) is reduced by each job's first workgroup
+  int iter, fin_n;
 };
 
 struct ConvArgs {

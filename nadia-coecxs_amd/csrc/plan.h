@@ -70,6 +70,11 @@ constexpr int row_points(int n) { return (n % 3 == 0) ? (n < 24 ? n : 24) : (n <
 // columns measured 4% slower).
 constexpr int col_points(int n) { return (n % 3 == 0 || n < 2048) ? row_points(n) : 16; }
 
+// LDS pad shift of the exchanges (one pad element per 2^shift positions):
+// column tiles of 16-point plans use 16, everything else 8.
+constexpr int col_pad_shift(int n) { return col_points(n) == 16 ? 4 : 3; }
+constexpr int row_pad_shift(int) { return 3; }
+
 // Transforms (rows) per row-pass workgroup: about 256 threads.
 constexpr int row_batch(int n) {
   int tpt = n / row_points(n);
