@@ -67,6 +67,7 @@ typedef struct nadia_recon nadia_recon; /* one reconstruction (BaseCDI object) *
 typedef struct nadia_batch nadia_batch; /* reconstructions iterated together */
 typedef struct nadia_constraint nadia_constraint; /* a TransmissionConstraint */
 typedef struct nadia_pd nadia_pd;                 /* a PhaseDiverseCDI */
+typedef struct nadia_event nadia_event;           /* a marker in the device's queue of work */
 /* host function applied to a transmission function (interleaved complex,
  * x-major, nx x ny, float or double as `precision`), in place */
 typedef void (*nadia_custom_constraint_fn)(void* complex_host, int nx, int ny, int precision, void* user);
@@ -317,6 +318,25 @@ int nadia_batch_iterate_each(nadia_batch* b, const int* counts);
 int nadia_batch_set_graph(nadia_batch* b, int steps);
 /* graph replays run so far (for tests and measurement) */
 int nadia_batch_graph_replays(const nadia_batch* b, long long* n);
+
+/* ---- asynchronous use (no counterpart in the reference) ----
+ * A request loop keeps the next batch of work queued while it reads the
+ * results of the last: events mark points in the library's queue on the
+ * current device, and get_error() can be copied out without waiting. */
+int nadia_event_create(nadia_event** out);
+int nadia_event_destroy(nadia_event* e);
+/* mark everything queued so far on the current device */
+int nadia_event_record(nadia_event* e);
+/* *done = 1 once the work before the last record has finished */
+int nadia_event_query(nadia_event* e, int* done);
+int nadia_event_synchronize(nadia_event* e);
+/* page-locked host memory (the destination of asynchronous reads) */
+int nadia_host_alloc(size_t bytes, void** out);
+int nadia_host_free(void* p);
+/* queue a copy of get_error() into *host (page-locked memory from
+ * nadia_host_alloc); the value is there once an event recorded after this
+ * call has completed */
+int nadia_recon_get_error_async(nadia_recon* r, double* host);
 
 /* ---- tuning and measurement ---- */
 

@@ -54,6 +54,14 @@ def _sig(lib):
         "nadia_set_device": (I, [I]),
         "nadia_size_supported": (I, [I]),
         "nadia_synchronize": (I, []),
+        "nadia_event_create": (I, [PP]),
+        "nadia_event_destroy": (I, [P]),
+        "nadia_event_record": (I, [P]),
+        "nadia_event_query": (I, [P, IP]),
+        "nadia_event_synchronize": (I, [P]),
+        "nadia_host_alloc": (I, [ctypes.c_size_t, PP]),
+        "nadia_host_free": (I, [P]),
+        "nadia_recon_get_error_async": (I, [P, P]),
         "nadia_field_create": (I, [I, I, I, PP]),
         "nadia_field_destroy": (I, [P]),
         "nadia_field_dims": (I, [P, IP, IP, IP]),
@@ -174,6 +182,52 @@ def _ptr(a):
 
 def synchronize():
     _check(library().nadia_synchronize())
+
+
+class Event:
+    """A marker in the library's queue of device work (no counterpart in the
+    reference): record() after queuing work, query() / synchronize() later."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        _check(library().nadia_event_create(ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.nadia_event_destroy(self.h)
+            self.h = None
+
+    def record(self):
+        _check(library().nadia_event_record(self.h))
+
+    def query(self):
+        done = ctypes.c_int(0)
+        _check(library().nadia_event_query(self.h, ctypes.byref(done)))
+        return bool(done.value)
+
+    def synchronize(self):
+        _check(library().nadia_event_synchronize(self.h))
+
+
+class PinnedDoubles:
+    """n doubles of page-locked host memory, as a numpy array (.a): the
+    destination of asynchronous reads such as BaseCDI.get_error_async."""
+
+    def __init__(self, n):
+        p = ctypes.c_void_p()
+        _check(library().nadia_host_alloc(8 * int(n), ctypes.byref(p)))
+        self.p = p
+        self.a = np.ctypeslib.as_array((ctypes.c_double * int(n)).from_address(p.value))
+
+    def __del__(self):
+        if getattr(self, "p", None) and _lib is not None:
+            self.a = None
+            _lib.nadia_host_free(self.p)
+            self.p = None
+
+    def ptr(self, i):
+        return ctypes.c_void_p(self.p.value + 8 * int(i))
 
 
 def get_alg_from_name(name):
@@ -326,6 +380,11 @@ class BaseCDI:
         v = ctypes.c_double()
         _check(library().nadia_recon_get_error(self.h, ctypes.byref(v)))
         return v.value
+
+    def get_error_async(self, pinned, index):
+        """Queue a copy of get_error() into pinned.a[index] (PinnedDoubles);
+        it is there once an Event recorded after this call has completed."""
+        _check(library().nadia_recon_get_error_async(self.h, pinned.ptr(index)))
 
     def apply_shrinkwrap(self, gauss_width=1.5, threshold=0.1):
         _check(library().nadia_recon_apply_shrinkwrap(self.h, float(gauss_width), float(threshold)))

@@ -214,7 +214,6 @@ struct nadia_recon {
   void* t[4];
   void* conv;
   void* rtmp;
-  void* gwin;
   double* partials;  // col-pass / reduction partial sums
   double* errs;      // ring of per-iteration errors
   double* scal;      // device scalars
@@ -1051,6 +1050,17 @@ int update_n_best(nadia_recon* r, const double* err, hipStream_t st) {
   return 0;
 }
 
+// BaseCDI::convolve's window g(i, j) = exp(-(i^2 + j^2) / (2 sigma^2)),
+// 0 <= i, j <= cut (src/BaseCDI.c++:441-447), rounded to the field's real
+// type as the reference stores it
+void conv_window(const nadia_recon* r, double sigma, int cut, double* g) {
+  for (int i = 0; i <= cut; i++)
+    for (int j = 0; j <= cut; j++) {
+      const double v = exp(-1 * (i * i + j * j) / (2.0 * sigma * sigma));
+      g[i * (cut + 1) + j] = r->prec == NADIA_FP64 ? v : (double)(float)v;
+    }
+}
+
 // set_support on a device real array already in r->rtmp (src/BaseCDI.c++:95-108)
 int support_from_rtmp(nadia_recon* r, int soften) {
   DISPATCH(r->prec, k_real_max<T>(r->rtmp, r->npix, r->partials, kEwBlocks, S_(r)));
@@ -1059,27 +1069,18 @@ int support_from_rtmp(nadia_recon* r, int soften) {
   LAUNCHCHK();
   if (soften) {
     // convolve(support, 3, 5): 11x11 window, sigma 3, left unnormalised
-    std::vector<double> g(36);
-    for (int i = 0; i <= 5; i++)
-      for (int j = 0; j <= 5; j++) g[i * 6 + j] = exp(-1 * (i * i + j * j) / (2.0 * 3 * 3));
-    std::vector<float> gf(g.begin(), g.end());
-    if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g.data(), 36 * 8, hipMemcpyHostToDevice, S_(r)));
-    else HIPCHK(hipMemcpyAsync(r->gwin, gf.data(), 36 * 4, hipMemcpyHostToDevice, S_(r)));
-    HIPCHK(hipStreamSynchronize(S_(r)));  // g / gf end with this scope
     ConvArgs ca{};
+    conv_window(r, 3.0, 5, ca.g);
     ca.src = r->S;
     ca.src_complex = 0;
     ca.dst = r->conv;
     ca.nx = r->nx;
     ca.ny = r->ny;
-    ca.g = r->gwin;
     ca.partials = r->partials;
     int nb = 0;
     DISPATCH(r->prec, k_conv<T>(ca, 5, &nb, S_(r)));
     LAUNCHCHK();
     HIPCHK(hipMemcpyAsync(r->S, r->conv, r->npix * r->rsz, hipMemcpyDeviceToDevice, S_(r)));
-    // the host window upload above must complete before `g` goes out of scope
-    HIPCHK(hipStreamSynchronize(S_(r)));
   }
   r->s_version++;
   return 0;
@@ -1311,7 +1312,7 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   const size_t cb = r->npix * r->esz, rb = r->npix * r->rsz;
   int rc = 0;
   if ((rc = alloc(&r->S, rb)) || (rc = alloc(&r->Q, rb)) || (rc = alloc(&r->W, 2 * cb)) ||
-      (rc = alloc(&r->conv, rb)) || (rc = alloc(&r->rtmp, rb)) || (rc = alloc(&r->gwin, 64 * 8)) ||
+      (rc = alloc(&r->conv, rb)) || (rc = alloc(&r->rtmp, rb)) ||
       (rc = alloc((void**)&r->partials, sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16))) ||
       (rc = alloc((void**)&r->errs, sizeof(double) * (r->ring_cap + 8))) ||
       (rc = alloc((void**)&r->scal, sizeof(double) * 16)) || (rc = alloc((void**)&r->djob, sizeof(JobDesc) * 2)) ||
@@ -1365,7 +1366,7 @@ namespace {
 // snapshots when present.
 size_t base_bytes(long long npix, size_t esz) {
   const size_t cb = npix * esz, rb = npix * (esz / 2);
-  return 4 * rb + 2 * cb + 64 * 8 + sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16) +
+  return 4 * rb + 2 * cb + sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16) +
          sizeof(double) * (4096 + 8) + sizeof(double) * 16 + sizeof(JobDesc) * 2 + sizeof(double) * 64;
 }
 }  // namespace
@@ -1402,7 +1403,7 @@ int nadia_recon_destroy(nadia_recon* r) {
   if (!r) return 0;
   (void)hipStreamSynchronize(S_(r));
   void* ptrs[] = {r->S, r->Q, r->beam, r->W, r->t[0], r->t[1], r->t[2], r->t[3], r->conv, r->rtmp,
-                  r->gwin, r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs, r->ttmp,
+                  r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs, r->ttmp,
                   r->dbest_err, r->dbest_perm, r->dbest_bufs, r->dbest_target, r->phx, r->phy};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1575,38 +1576,24 @@ int nadia_recon_get_error(nadia_recon* r, double* out) {
 
 int nadia_recon_apply_shrinkwrap(nadia_recon* r, double gauss_width, double threshold) {
   if (!r) return fail(NADIA_EINVAL, "null recon");
-  // window g(i,j) = exp(-(i^2+j^2)/(2 sigma^2)), i,j in [0,4], stored in the
-  // field's real type (BaseCDI::convolve, src/BaseCDI.c++:441-447)
-  double g[25];
-  for (int i = 0; i <= 4; i++)
-    for (int j = 0; j <= 4; j++) {
-      double denom = 2.0 * gauss_width * gauss_width;
-      g[i * 5 + j] = exp(-1 * (i * i + j * j) / denom);
-    }
-  float gf[25];
-  for (int i = 0; i < 25; i++) gf[i] = (float)g[i];
-  if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g, sizeof(g), hipMemcpyHostToDevice, S_(r)));
-  else HIPCHK(hipMemcpyAsync(r->gwin, gf, sizeof(gf), hipMemcpyHostToDevice, S_(r)));
-  // the window lives on this stack frame: the copy must be done before the
-  // function returns (a pageable host copy may be read after the call)
-  HIPCHK(hipStreamSynchronize(S_(r)));
+  // BaseCDI::apply_shrinkwrap (src/BaseCDI.c++:416-429): convolve |x| with
+  // the 9x9 window, threshold at thr * max, set_support.  Everything is
+  // queued on the device (the window travels in the kernel arguments): the
+  // call returns without waiting.
   ConvArgs ca{};
+  conv_window(r, gauss_width, 4, ca.g);
   ca.src = r->est->d;
   ca.src_complex = 1;
   ca.dst = r->conv;
   ca.nx = r->nx;
   ca.ny = r->ny;
-  ca.g = r->gwin;
   ca.partials = r->partials;
   int nb = 0;
   DISPATCH(r->prec, k_conv<T>(ca, 4, &nb, S_(r)));
   k_final_max(r->partials, nb, r->scal, S_(r));
   DISPATCH(r->prec, k_threshold<T>(r->conv, r->rtmp, r->nx, r->ny, r->scal, threshold, S_(r)));
   LAUNCHCHK();
-  int rc = support_from_rtmp(r, 0);
-  if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(S_(r)));  // g / gf live on this stack frame
-  return 0;
+  return support_from_rtmp(r, 0);
 }
 
 int nadia_recon_get_support(const nadia_recon* r, void* host) {
@@ -1997,20 +1984,13 @@ int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size) {
   LAUNCHCHK();
   // convolve(support, 3, 5), left unnormalised
   HIPCHK(hipMemcpyAsync(r->rtmp, r->S, r->npix * r->rsz, hipMemcpyDeviceToDevice, S_(r)));
-  std::vector<double> g(36);
-  for (int i = 0; i <= 5; i++)
-    for (int j = 0; j <= 5; j++) g[i * 6 + j] = exp(-1 * (i * i + j * j) / (2.0 * 3 * 3));
-  std::vector<float> gf(g.begin(), g.end());
-  if (r->prec == NADIA_FP64) HIPCHK(hipMemcpyAsync(r->gwin, g.data(), 36 * 8, hipMemcpyHostToDevice, S_(r)));
-  else HIPCHK(hipMemcpyAsync(r->gwin, gf.data(), 36 * 4, hipMemcpyHostToDevice, S_(r)));
-  HIPCHK(hipStreamSynchronize(S_(r)));  // g / gf end with this function
   ConvArgs ca{};
+  conv_window(r, 3.0, 5, ca.g);
   ca.src = r->rtmp;
   ca.src_complex = 0;
   ca.dst = r->S;
   ca.nx = r->nx;
   ca.ny = r->ny;
-  ca.g = r->gwin;
   ca.partials = r->partials;
   int nb = 0;
   DISPATCH(r->prec, k_conv<T>(ca, 5, &nb, S_(r)));
@@ -3485,6 +3465,92 @@ int nadia_batch_iterate_each(nadia_batch* b, const int* counts) {
   if (!b || (!counts && !b->jobs.empty())) return fail(NADIA_EINVAL, "null argument");
   if (b->jobs.empty()) return 0;
   return batch_run(b, std::vector<int>(counts, counts + b->jobs.size()), nullptr);
+}
+
+// ---- asynchronous use ------------------------------------------------------
+
+}  // extern "C"
+
+struct nadia_event {
+  int dev;
+  hipEvent_t ev;
+  bool recorded;
+};
+
+extern "C" {
+
+int nadia_event_create(nadia_event** out) {
+  if (!out) return fail(NADIA_EINVAL, "null output pointer");
+  nadia_event* e = new nadia_event{};
+  e->dev = current_device();
+  if (!dev_ctx(e->dev)) {
+    delete e;
+    return fail(NADIA_EHIP, "stream creation failed");
+  }
+  if (hipEventCreateWithFlags(&e->ev, hipEventDisableTiming) != hipSuccess) {
+    delete e;
+    return fail(NADIA_EHIP, "hipEventCreateWithFlags failed");
+  }
+  *out = e;
+  return 0;
+}
+
+int nadia_event_destroy(nadia_event* e) {
+  if (!e) return 0;
+  (void)hipEventDestroy(e->ev);
+  delete e;
+  return 0;
+}
+
+int nadia_event_record(nadia_event* e) {
+  if (!e) return fail(NADIA_EINVAL, "null event");
+  HIPCHK(hipEventRecord(e->ev, dev_ctx(e->dev)->stream));
+  e->recorded = true;
+  return 0;
+}
+
+int nadia_event_query(nadia_event* e, int* done) {
+  if (!e || !done) return fail(NADIA_EINVAL, "null argument");
+  if (!e->recorded) {
+    *done = 1;
+    return 0;
+  }
+  const hipError_t q = hipEventQuery(e->ev);
+  if (q == hipSuccess) *done = 1;
+  else if (q == hipErrorNotReady) *done = 0;
+  else return fail(NADIA_EHIP, "hipEventQuery: %s", hipGetErrorString(q));
+  return 0;
+}
+
+int nadia_event_synchronize(nadia_event* e) {
+  if (!e) return fail(NADIA_EINVAL, "null event");
+  if (e->recorded) HIPCHK(hipEventSynchronize(e->ev));
+  return 0;
+}
+
+int nadia_host_alloc(size_t bytes, void** out) {
+  if (!out) return fail(NADIA_EINVAL, "null output pointer");
+  HIPCHK(hipHostMalloc(out, bytes ? bytes : 8, hipHostMallocDefault));
+  return 0;
+}
+
+int nadia_host_free(void* p) {
+  if (p) HIPCHK(hipHostFree(p));
+  return 0;
+}
+
+int nadia_recon_get_error_async(nadia_recon* r, double* host) {
+  if (!r || !host) return fail(NADIA_EINVAL, "null argument");
+  if (r->h_err_valid) {
+    *host = r->h_err_cache;
+    return 0;
+  }
+  if (!r->last_err) {  // never projected: get_error() reports 0
+    *host = 0.0;
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(host, r->last_err, sizeof(double), hipMemcpyDeviceToHost, S_(r)));
+  return 0;
 }
 
 int nadia_tile_variant_count(int* col_variants, int* row_variants) {

@@ -103,7 +103,9 @@ struct ConvArgs {
   int src_complex;
   void* dst;        // real
   int nx, ny;
-  const void* g;    // (cut+1)^2 window in the field's real type
+  // (cut+1)^2 window values, rounded to the field's real type, passed by
+  // value (no upload, nothing for the host to keep alive)
+  double g[36];
   double* partials; // one max per block
 };
 

@@ -15,6 +15,10 @@ schedule.  The server
   (nadia_batch_iterate_each);
 * retires a job as soon as its last iteration is done, and admits waiting
   requests in the memory and slots it frees, while the others keep running.
+* With a backend that can mark its queue (mark / done / wait_mark, retire /
+  collect), the device never waits for the host: tick k+1 is queued before
+  the results of tick k are read (asynchronously, into page-locked memory),
+  so retiring, admitting and scheduling overlap the device's work.
 
 A job's result does not depend on what it ran beside: batched launches
 compute every job independently, with the same kernels and reductions as a
@@ -95,7 +99,7 @@ def percentile(values, q):
 
 
 class _Job:
-    __slots__ = ("req", "handle", "bytes", "done", "calls", "ncalls", "admitted")
+    __slots__ = ("req", "handle", "bytes", "done", "calls", "ncalls", "admitted", "pending")
 
     def __init__(self, req, handle, nbytes, admitted, every):
         self.req = req
@@ -114,6 +118,10 @@ class Server:
     run(handles, counts), shrinkwrap(handle, sigma, threshold), wait(),
     finish(handle, keep) -> (error, result), release_idle(nbytes) -> freed
     bytes (optional), held_bytes() -> bytes of idle cached jobs (optional).
+    Asynchronous backends add mark() -> token (after the queued work),
+    done(token), wait_mark(token), retire(handle, keep) -> pending (the job
+    leaves the running set; its results are read asynchronously) and
+    collect(pending) -> (error, result) once a later mark has completed.
     """
 
     def __init__(self, backend, hbm_budget=200e9, max_active=None, tick=50, every=50, threshold=0.1,
@@ -142,6 +150,8 @@ class Server:
         done = []
         t0 = clock()
         be = self.backend
+        if hasattr(be, "mark"):
+            return self._run_async(pending, waiting, clock, sleep)
         while pending or waiting or active:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
@@ -199,6 +209,92 @@ class Server:
         self.elapsed = clock() - t0
         return done
 
+    def _admit(self, waiting, active, used, t0, clock):
+        be = self.backend
+        while waiting and (self.max_active is None or len(active) < self.max_active):
+            req = waiting[0]
+            need = be.bytes_for(req)
+            if need > self.budget:
+                raise ValueError("request %d needs %d bytes, more than the whole budget" % (req.rid, need))
+            if used + self._held() + need > self.budget and hasattr(be, "release_idle"):
+                be.release_idle(used + self._held() + need - self.budget)
+            if used + self._held() + need > self.budget:
+                break
+            waiting.popleft()
+            h = be.start(req)
+            active.append(_Job(req, h, need, clock() - t0, self.every))
+            used += need
+        return used
+
+    def _collect(self, entry, done, t0, clock):
+        """Wait for a queued tick and record its finished jobs; returns the
+        bytes they free."""
+        tok, fin = entry
+        self.backend.wait_mark(tok)
+        tf = clock() - t0
+        freed = 0
+        for j in fin:
+            err, res = self.backend.collect(j.pending)
+            done.append(Completed(j.req.rid, j.req.n, j.req.arrival, j.admitted, tf, j.done, err, res))
+            freed += j.bytes
+        return freed
+
+    def _run_async(self, pending, waiting, clock, sleep):
+        """The request loop over an asynchronous backend: at most two ticks
+        are queued; the older one's finished jobs are read while the device
+        runs the newer one."""
+        be = self.backend
+        active, done = [], []
+        inflight = collections.deque()  # (token, jobs that finished in that tick)
+        used = 0
+        t0 = clock()
+        while pending or waiting or active or inflight:
+            now = clock() - t0
+            while pending and pending[0].arrival <= now:
+                waiting.append(pending.popleft())
+            used = self._admit(waiting, active, used, t0, clock)
+            if waiting and any(fin for _, fin in inflight):
+                # admission is blocked (memory or slots) and queued ticks
+                # hold finished jobs: free them first, then admit again
+                while inflight:
+                    used -= self._collect(inflight.popleft(), done, t0, clock)
+                used = self._admit(waiting, active, used, t0, clock)
+            self.peak_bytes = max(self.peak_bytes, used + self._held())
+            self.peak_active = max(self.peak_active, len(active))
+            if active:
+                counts = []
+                for j in active:
+                    left = j.req.iterations - j.done
+                    c = min(self.tick, left)
+                    if j.ncalls and j.calls < j.ncalls:
+                        c = min(c, (j.calls + 1) * self.every - j.done)
+                    counts.append(c)
+                be.run([j.handle for j in active], counts)
+                self.ticks += 1
+                self.iterations += sum(counts)
+                for j, c in zip(active, counts):
+                    j.done += c
+                    if j.ncalls and j.calls < j.ncalls and j.done == (j.calls + 1) * self.every:
+                        be.shrinkwrap(j.handle, shrinkwrap_sigma(j.calls, j.ncalls), self.threshold)
+                        j.calls += 1
+                finished = [j for j in active if j.done >= j.req.iterations]
+                for j in finished:
+                    j.pending = be.retire(j.handle, self.keep)
+                inflight.append((be.mark(), finished))
+                active = [j for j in active if j.done < j.req.iterations]
+            # read the older tick while the newer one runs; wait for the last
+            # one only when nothing else is queued behind it
+            while inflight and (len(inflight) > 1 or not active or be.done(inflight[0][0])):
+                used -= self._collect(inflight.popleft(), done, t0, clock)
+                if active:
+                    break
+            if not active and not inflight and not waiting and pending:
+                dt = pending[0].arrival - (clock() - t0)
+                if dt > 0:
+                    sleep(dt)
+        self.elapsed = clock() - t0
+        return done
+
     @staticmethod
     def summary(done, elapsed):
         lat = [c.latency * 1e3 for c in done]
@@ -228,6 +324,9 @@ class DeviceBackend:
         self.idle = collections.defaultdict(list)
         self.idle_bytes = 0
         self._bytes = {}
+        self._events = []       # recorded-and-waited events, reused
+        self._pinned = []       # PinnedDoubles blocks of error slots
+        self._free_slots = []
 
     def bytes_for(self, req):
         if req.n not in self._bytes:
@@ -285,6 +384,41 @@ class DeviceBackend:
 
     def wait(self):
         self.nd.synchronize()
+
+    # ---- asynchronous retirement ----
+    def mark(self):
+        ev = self._events.pop() if self._events else self.nd.Event()
+        ev.record()
+        return ev
+
+    def done(self, ev):
+        return ev.query()
+
+    def wait_mark(self, ev):
+        ev.synchronize()
+        self._events.append(ev)
+
+    def retire(self, handle, keep):
+        """The job leaves the batch now (the next tick does not run it); its
+        error is copied out asynchronously into a page-locked slot."""
+        if not self._free_slots:
+            blk = self.nd.PinnedDoubles(256)
+            self._pinned.append(blk)
+            self._free_slots.extend((blk, i) for i in range(256))
+        slot = self._free_slots.pop()
+        handle[2].get_error_async(slot[0], slot[1])
+        self.batch.remove(handle[2])
+        return (handle, slot, keep)
+
+    def collect(self, pending):
+        handle, slot, keep = pending
+        req, est, p, beam = handle
+        err = float(slot[0].a[slot[1]])
+        self._free_slots.append(slot)
+        res = est.get() if keep else None
+        self.idle[(req.n, beam)].append((est, p))
+        self.idle_bytes += self._bytes[req.n]
+        return err, res
 
     def finish(self, handle, keep):
         req, est, p, beam = handle

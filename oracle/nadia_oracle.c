@@ -167,18 +167,33 @@ void or_fft2d(cplx *a, int nx, int ny, int sign) {
   plan1_make(&px, nx);
   int nmax = nx > ny ? nx : ny;
   const int B = 8; /* columns gathered per block */
-  cplx *buf = (cplx *)malloc(sizeof(cplx) * (size_t)nmax * B);
-  cplx *scr = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
-  for (int i = 0; i < nx; i++) fft1(&py, a + (size_t)i * ny, scr, sign);
-  for (int j0 = 0; j0 < ny; j0 += B) {
-    int nb = (ny - j0) < B ? (ny - j0) : B;
-    for (int i = 0; i < nx; i++)
-      for (int b = 0; b < nb; b++) buf[(size_t)b * nx + i] = a[(size_t)i * ny + j0 + b];
-    for (int b = 0; b < nb; b++) fft1(&px, buf + (size_t)b * nx, scr, sign);
-    for (int i = 0; i < nx; i++)
-      for (int b = 0; b < nb; b++) a[(size_t)i * ny + j0 + b] = buf[(size_t)b * nx + i];
+  /* The timing build (ORACLE_OMP) splits the row transforms and the column
+   * blocks over threads, the way FFTW's threaded plans split a 2-D
+   * transform (NADIA's -DMULTI_THREADED build, NADIA/src/Complex_2D.c++:465-468);
+   * each transform's arithmetic is unchanged. */
+#ifdef ORACLE_OMP
+#pragma omp parallel
+#endif
+  {
+    cplx *buf = (cplx *)malloc(sizeof(cplx) * (size_t)nmax * B);
+    cplx *scr = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+#ifdef ORACLE_OMP
+#pragma omp for schedule(static)
+#endif
+    for (int i = 0; i < nx; i++) fft1(&py, a + (size_t)i * ny, scr, sign);
+#ifdef ORACLE_OMP
+#pragma omp for schedule(static)
+#endif
+    for (int j0 = 0; j0 < ny; j0 += B) {
+      int nb = (ny - j0) < B ? (ny - j0) : B;
+      for (int i = 0; i < nx; i++)
+        for (int b = 0; b < nb; b++) buf[(size_t)b * nx + i] = a[(size_t)i * ny + j0 + b];
+      for (int b = 0; b < nb; b++) fft1(&px, buf + (size_t)b * nx, scr, sign);
+      for (int i = 0; i < nx; i++)
+        for (int b = 0; b < nb; b++) a[(size_t)i * ny + j0 + b] = buf[(size_t)b * nx + i];
+    }
+    free(buf); free(scr);
   }
-  free(buf); free(scr);
   plan1_free(&px); plan1_free(&py);
 }
 

@@ -24,8 +24,10 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-def _lib(double=True):
-    key = "d" if double else "f"
+def _lib(double=True, timing=False):
+    """The oracle library; timing=True selects the CPU-baseline builds
+    (-O3 -ffast-math; FP32 with threaded 2-D transforms), never the checker."""
+    key = ("d" if double else "f") + ("t" if timing else "")
     if key in _libs:
         return _libs[key]
     path = os.path.join(HERE, "liboracle_%s.so" % key)
@@ -189,12 +191,12 @@ class OracleConstraint:
 class OracleCDI:
     """One BaseCDI-derived object (PlanarCDI, FresnelCDI or FresnelCDI_WF)."""
 
-    def __init__(self, estimate, kind=PLANAR, n_best=0, double=True):
+    def __init__(self, estimate, kind=PLANAR, n_best=0, double=True, timing=False):
         self.double = double
         self.cdt = np.complex128 if double else np.complex64
         self.rdt = np.float64 if double else np.float32
         assert estimate.dtype == self.cdt and estimate.ndim == 2
-        self.lib = _lib(double)
+        self.lib = _lib(double, timing)
         self.nx, self.ny = estimate.shape
         self.complex = estimate  # mutated in place, like BaseCDI::complex
         self.h = self.lib.ocdi_create(_ptr(estimate), self.nx, self.ny, kind, n_best)

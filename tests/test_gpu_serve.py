@@ -108,3 +108,18 @@ def test_served_results_match_alone(nd, synth):
         g.iterate_async(r.iterations - it)
         assert np.array_equal(est.get(), c.result), c.rid
         assert g.get_error() == c.error, c.rid
+
+
+def test_event_and_async_error(nd, synth):
+    """get_error copied out asynchronously into page-locked memory equals
+    get_error() once an event recorded after it has completed."""
+    est, g = planar_job(nd, synth, 256, 45, nd.HIO, double=False)
+    buf = nd.PinnedDoubles(4)
+    ev = nd.Event()
+    assert ev.query()  # never recorded: nothing to wait for
+    g.iterate_async(30)
+    g.get_error_async(buf, 2)
+    ev.record()
+    ev.synchronize()
+    assert ev.query()
+    assert buf.a[2] == g.get_error() and 0 < buf.a[2] < 1

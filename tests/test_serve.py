@@ -24,9 +24,10 @@ class FakeBackend:
     """Each launch costs `cost` seconds per Mpx of the largest job in it;
     jobs in one tick share launches, so a tick costs its largest count."""
 
-    def __init__(self, clock, cost=1e-3, bytes_per_px=100):
+    def __init__(self, clock, cost=1e-3, bytes_per_px=100, host_cost=0.0):
         self.clock = clock
         self.cost = cost
+        self.host_cost = host_cost  # host seconds to queue one tick
         self.bpp = bytes_per_px
         self.live = {}
         self.used = 0
@@ -53,7 +54,7 @@ class FakeBackend:
         for h, c in zip(handles, counts):
             h["done"] += c
         mpx = max(h["req"].n ** 2 for h in handles) / 1e6
-        self.clock.t += max(counts) * self.cost * mpx
+        self.clock.t += self.host_cost + max(counts) * self.cost * mpx
 
     def shrinkwrap(self, h, sigma, thr):
         self.shrinks[h["req"].rid].append((h["done"], sigma, thr))
@@ -67,12 +68,57 @@ class FakeBackend:
         return float(h["done"]), None
 
 
-def run(reqs, **kw):
+class FakeAsyncBackend(FakeBackend):
+    """The device as a queue: run() queues work that finishes `cost` later
+    on the device timeline (the host clock does not wait); mark() is the
+    completion time of everything queued; retired jobs must not run again
+    and at most two ticks may be queued ahead of the host."""
+
+    def __init__(self, clock, **kw):
+        super().__init__(clock, **kw)
+        self.device_free = 0.0
+        self.retired = set()
+        self.marks = []
+
+    def run(self, handles, counts):
+        assert not any(h["req"].rid in self.retired for h in handles)
+        self.counts_seen.append(list(counts))
+        for h, c in zip(handles, counts):
+            h["done"] += c
+        mpx = max(h["req"].n ** 2 for h in handles) / 1e6
+        self.clock.t += self.host_cost
+        self.device_free = max(self.device_free, self.clock.t) + max(counts) * self.cost * mpx
+
+    def mark(self):
+        self.marks.append(self.device_free)
+        pending = [m for m in self.marks if m > self.clock.t]
+        assert len(pending) <= 2, "more than two ticks queued ahead of the host"
+        return self.device_free
+
+    def done(self, tok):
+        return self.clock.t >= tok
+
+    def wait_mark(self, tok):
+        self.clock.t = max(self.clock.t, tok)
+
+    def retire(self, h, keep):
+        self.retired.add(h["req"].rid)
+        return h
+
+    def collect(self, h):
+        assert self.clock.t >= 0
+        return self.finish(h, False)
+
+
+def run(reqs, backend=FakeBackend, host_cost=0.0, **kw):
     clock = Clock()
-    be = FakeBackend(clock)
+    be = backend(clock, host_cost=host_cost)
     srv = serve.Server(be, **kw)
     done = srv.run(reqs, clock=clock, sleep=clock.sleep)
     return srv, be, done
+
+
+BACKENDS = pytest.mark.parametrize("backend", [FakeBackend, FakeAsyncBackend], ids=["sync", "async"])
 
 
 def test_stream_is_seeded_and_follows_the_distribution():
@@ -113,11 +159,12 @@ def test_every_request_completes_with_its_iterations_and_schedule():
     assert srv.iterations == sum(r.iterations for r in reqs)
 
 
-def test_admission_respects_the_budget_in_arrival_order():
+@BACKENDS
+def test_admission_respects_the_budget_in_arrival_order(backend):
     reqs = [serve.Request(i, 0.001 * i, 512 if i % 3 else 1024, serve.HIO, 0.9, 100 + 10 * i, False, i)
             for i in range(30)]
     budget = 4 * 1024 * 1024 * 100  # room for 4 jobs of 1024^2 (or 16 of 512^2)
-    srv, be, done = run(reqs, hbm_budget=budget)
+    srv, be, done = run(reqs, backend, hbm_budget=budget)
     assert be.peak <= budget and srv.peak_bytes <= budget
     assert be.started == sorted(be.started)  # first come, first admitted
     assert len(done) == 30
@@ -125,10 +172,11 @@ def test_admission_respects_the_budget_in_arrival_order():
     assert all(adm[i] <= adm[i + 1] for i in range(29))
 
 
-def test_continuous_batching_admits_while_others_run():
+@BACKENDS
+def test_continuous_batching_admits_while_others_run(backend):
     long_job = serve.Request(0, 0.0, 1024, serve.HIO, 0.9, 1000, False, 0)
     short = [serve.Request(i, 0.05 * i, 256, serve.ER, 0.9, 100, False, i) for i in range(1, 6)]
-    srv, be, done = run([long_job] + short, hbm_budget=1e12)
+    srv, be, done = run([long_job] + short, backend, hbm_budget=1e12)
     fin = {c.rid: c for c in done}
     for i in range(1, 6):
         # admitted and finished while the long job is still running
@@ -137,19 +185,37 @@ def test_continuous_batching_admits_while_others_run():
     assert srv.peak_active >= 2
 
 
-def test_slot_limit_and_oversized_request():
+@BACKENDS
+def test_slot_limit_and_oversized_request(backend):
     reqs = [serve.Request(i, 0.0, 256, serve.ER, 0.9, 100, False, i) for i in range(8)]
-    srv, be, done = run(reqs, hbm_budget=1e12, max_active=3)
+    srv, be, done = run(reqs, backend, hbm_budget=1e12, max_active=3)
     assert srv.peak_active == 3 and len(done) == 8
     with pytest.raises(ValueError):
-        run([serve.Request(0, 0.0, 2048, serve.ER, 0.9, 100, False, 0)], hbm_budget=1e6)
+        run([serve.Request(0, 0.0, 2048, serve.ER, 0.9, 100, False, 0)], backend, hbm_budget=1e6)
 
 
-def test_summary_percentiles():
+@BACKENDS
+def test_summary_percentiles(backend):
     reqs = serve.poisson_stream(count=40, rate=100.0, sizes=(256,), probs=(1.0,), iterations=(100, 200), seed=9)
-    srv, be, done = run(reqs, hbm_budget=1e12)
+    srv, be, done = run(reqs, backend, hbm_budget=1e12)
     s = serve.Server.summary(done, srv.elapsed)
     lat = sorted(c.latency * 1e3 for c in done)
     assert s["requests"] == 40
     assert lat[0] <= s["p50_ms"] <= s["p99_ms"] <= lat[-1]
     assert s["aggregate_iter_s"] > 0
+
+
+def test_async_loop_matches_sync_schedule():
+    """Same requests through both loops: every job gets the same iteration
+    counts and shrinkwrap points; with host work per tick (4 ms) the
+    asynchronous loop, which overlaps it with the device, finishes no later."""
+    reqs = serve.poisson_stream(count=60, rate=200.0, sizes=(256, 512, 1024), probs=(0.5, 0.3, 0.2),
+                                iterations=(60, 400), seed=11)
+    s1, b1, d1 = run(reqs, FakeBackend, host_cost=4e-3, hbm_budget=6 * 1024 * 1024 * 100)
+    s2, b2, d2 = run(reqs, FakeAsyncBackend, host_cost=4e-3, hbm_budget=6 * 1024 * 1024 * 100)
+    assert sorted(c.rid for c in d1) == sorted(c.rid for c in d2) == list(range(60))
+    for r in reqs:
+        assert b1.shrinks[r.rid] == b2.shrinks[r.rid]
+    assert {c.rid: c.iterations for c in d1} == {c.rid: c.iterations for c in d2}
+    assert s2.elapsed <= s1.elapsed + 1e-9
+    assert s2.peak_bytes <= 6 * 1024 * 1024 * 100
