@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <mutex>
 #include <string>
@@ -57,8 +58,27 @@ int fail(int code, const char* fmt, ...) {
     if (e_ != hipSuccess) return fail(NADIA_EHIP, "kernel launch: %s", hipGetErrorString(e_)); \
   } while (0)
 
+// Page-locked staging for uploads from the caller's (pageable) host arrays:
+// the array is copied into it on the host, the device copy is queued from it
+// and the call returns without waiting for the device.  Regions are handed
+// out in ring order and reused only after the copies from them completed.
+constexpr size_t kStageBytes = (size_t)256 << 20;
+
+struct Staging {
+  char* base = nullptr;
+  size_t head = 0;
+  struct Use {
+    size_t off, len;
+    hipEvent_t ev;
+  };
+  std::deque<Use> uses;            // in-flight copies, oldest first
+  std::vector<hipEvent_t> spare;   // completed events for reuse
+};
+
 struct DeviceCtx {
   hipStream_t stream = nullptr;
+  Staging stage;
+  std::mutex stage_mu;
   // this device's address of the file-scope `scratch` array (set once, by
   // the first allocation-free call on the device)
   void* scratch = nullptr;
@@ -86,6 +106,43 @@ DeviceCtx* dev_ctx(int dev) {
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
   }
   return &c;
+}
+
+// Queue a copy of `bytes` of pageable host memory to device memory `dst` on
+// the device's stream and return without waiting for the device (through
+// the staging ring; uploads larger than half of it copy directly and wait).
+int upload(DeviceCtx* c, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return 0;
+  if (bytes > kStageBytes / 2) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+  }
+  std::lock_guard<std::mutex> lk(c->stage_mu);
+  Staging& st = c->stage;
+  if (!st.base) HIPCHK(hipHostMalloc((void**)&st.base, kStageBytes, hipHostMallocDefault));
+  const size_t len = (bytes + 255) & ~(size_t)255;
+  size_t off = st.head;
+  if (off + len > kStageBytes) off = 0;
+  auto overlaps = [&](const Staging::Use& u) { return u.off < off + len && off < u.off + u.len; };
+  while (std::any_of(st.uses.begin(), st.uses.end(), overlaps)) {
+    HIPCHK(hipEventSynchronize(st.uses.front().ev));
+    st.spare.push_back(st.uses.front().ev);
+    st.uses.pop_front();
+  }
+  hipEvent_t ev;
+  if (!st.spare.empty()) {
+    ev = st.spare.back();
+    st.spare.pop_back();
+  } else {
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  std::memcpy(st.base + off, src, bytes);
+  HIPCHK(hipMemcpyAsync(dst, st.base + off, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipEventRecord(ev, c->stream));
+  st.uses.push_back({off, len, ev});
+  st.head = off + len;
+  return 0;
 }
 
 // Twiddle table of the Stockham plan for length n: per stage s, L*(r-1)
@@ -208,6 +265,7 @@ struct nadia_recon {
   DeviceCtx* ctx;
   size_t esz, rsz;
   long long npix;
+  void* arena;  // one allocation holding S, Q, W, conv, rtmp and the small per-job buffers
   void *S, *Q, *beam;
   int has_beam, has_intensity;
   void* W;  // 2 fields
@@ -1142,7 +1200,6 @@ int nadia_field_create(int nx, int ny, int precision, nadia_field** out) {
     return fail(NADIA_EHIP, "hipMalloc of %dx%d field failed", nx, ny);
   }
   HIPCHK(hipMemsetAsync(f->d, 0, (size_t)nx * ny * f->esz, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
   *out = f;
   return 0;
 }
@@ -1166,9 +1223,8 @@ int nadia_field_dims(const nadia_field* f, int* nx, int* ny, int* precision) {
 
 int nadia_field_upload(nadia_field* f, const void* host) {
   if (!f || !host) return fail(NADIA_EINVAL, "null argument");
-  DeviceCtx* c = dev_ctx(f->dev);
-  HIPCHK(hipMemcpyAsync(f->d, host, (size_t)f->nx * f->ny * f->esz, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc;
+  if ((rc = upload(dev_ctx(f->dev), f->d, host, (size_t)f->nx * f->ny * f->esz))) return rc;
   f->version++;
   return 0;
 }
@@ -1311,14 +1367,25 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   r->tile_fixed = false;
   const size_t cb = r->npix * r->esz, rb = r->npix * r->rsz;
   int rc = 0;
-  if ((rc = alloc(&r->S, rb)) || (rc = alloc(&r->Q, rb)) || (rc = alloc(&r->W, 2 * cb)) ||
-      (rc = alloc(&r->conv, rb)) || (rc = alloc(&r->rtmp, rb)) ||
-      (rc = alloc((void**)&r->partials, sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16))) ||
-      (rc = alloc((void**)&r->errs, sizeof(double) * (r->ring_cap + 8))) ||
-      (rc = alloc((void**)&r->scal, sizeof(double) * 16)) || (rc = alloc((void**)&r->djob, sizeof(JobDesc) * 2)) ||
-      (rc = alloc((void**)&r->tune_errs, sizeof(double) * 64))) {
-    delete r;
-    return rc;
+  // the fixed buffers share one allocation (one hipMalloc per
+  // reconstruction), each on a 256-byte boundary
+  {
+    void** dst[] = {&r->S, &r->Q, &r->W, &r->conv, &r->rtmp, (void**)&r->partials, (void**)&r->errs,
+                    (void**)&r->scal, (void**)&r->djob, (void**)&r->tune_errs};
+    const size_t len[] = {rb, rb, 2 * cb, rb, rb, sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16),
+                          sizeof(double) * (r->ring_cap + 8), sizeof(double) * 16, sizeof(JobDesc) * 2,
+                          sizeof(double) * 64};
+    size_t total = 0;
+    for (size_t b : len) total += (b + 255) & ~(size_t)255;
+    if ((rc = alloc(&r->arena, total))) {
+      delete r;
+      return rc;
+    }
+    char* p = (char*)r->arena;
+    for (int i = 0; i < 10; i++) {
+      *dst[i] = p;
+      p += (len[i] + 255) & ~(size_t)255;
+    }
   }
   (void)hipMemsetAsync(r->S, 0, rb, S_(r));
   (void)hipMemsetAsync(r->Q, 0, rb, S_(r));
@@ -1349,7 +1416,6 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
     // the host arrays above end with this block: the copies must be done
     HIPCHK(hipStreamSynchronize(S_(r)));
   }
-  HIPCHK(hipStreamSynchronize(S_(r)));
   // BaseCDI::BaseCDI: algorithm = CUSTOM; set_algorithm(ER) (src/BaseCDI.c++:63-64)
   r->algorithm = NADIA_CUSTOM;
   nadia_recon_set_algorithm(r, NADIA_ER);
@@ -1366,8 +1432,9 @@ namespace {
 // snapshots when present.
 size_t base_bytes(long long npix, size_t esz) {
   const size_t cb = npix * esz, rb = npix * (esz / 2);
+  // (+ the 256-byte alignment of the ten buffers of the shared allocation)
   return 4 * rb + 2 * cb + sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16) +
-         sizeof(double) * (4096 + 8) + sizeof(double) * 16 + sizeof(JobDesc) * 2 + sizeof(double) * 64;
+         sizeof(double) * (4096 + 8) + sizeof(double) * 16 + sizeof(JobDesc) * 2 + sizeof(double) * 64 + 10 * 256;
 }
 }  // namespace
 
@@ -1402,8 +1469,7 @@ int nadia_recon_bytes_estimate(int nx, int ny, int precision, int kind, int n_be
 int nadia_recon_destroy(nadia_recon* r) {
   if (!r) return 0;
   (void)hipStreamSynchronize(S_(r));
-  void* ptrs[] = {r->S, r->Q, r->beam, r->W, r->t[0], r->t[1], r->t[2], r->t[3], r->conv, r->rtmp,
-                  r->partials, r->errs, r->scal, r->L, r->Ls, r->djob, r->tune_errs, r->ttmp,
+  void* ptrs[] = {r->arena, r->beam, r->t[0], r->t[1], r->t[2], r->t[3], r->L, r->Ls, r->ttmp,
                   r->dbest_err, r->dbest_perm, r->dbest_bufs, r->dbest_target, r->phx, r->phy};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1419,37 +1485,35 @@ int nadia_recon_set_path(nadia_recon* r, int path) {
   return 0;
 }
 
+// The setters copy the caller's array (set_* copy their inputs,
+// src/BaseCDI.c++:95-124) into the staging ring and queue the rest on the
+// device: they return without waiting for it.
 int nadia_recon_set_support(nadia_recon* r, const void* host, int soften) {
   if (!r || !host) return fail(NADIA_EINVAL, "null argument");
-  HIPCHK(hipMemcpyAsync(r->rtmp, host, r->npix * r->rsz, hipMemcpyHostToDevice, S_(r)));
-  int rc = support_from_rtmp(r, soften);
-  if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(S_(r)));
-  return 0;
+  int rc;
+  if ((rc = upload(r->ctx, r->rtmp, host, r->npix * r->rsz))) return rc;
+  return support_from_rtmp(r, soften);
 }
 
 int nadia_recon_set_intensity(nadia_recon* r, const void* host) {
   if (!r || !host) return fail(NADIA_EINVAL, "null argument");
-  HIPCHK(hipMemcpyAsync(r->rtmp, host, r->npix * r->rsz, hipMemcpyHostToDevice, S_(r)));
+  int rc;
+  if ((rc = upload(r->ctx, r->rtmp, host, r->npix * r->rsz))) return rc;
   DISPATCH(r->prec, k_make_q<T>(r->rtmp, r->beam, r->has_beam, r->Q, r->npix, S_(r)));
   LAUNCHCHK();
-  HIPCHK(hipStreamSynchronize(S_(r)));
   r->has_intensity = 1;
   return 0;
 }
 
 int nadia_recon_set_beam_stop(nadia_recon* r, const void* host) {
   if (!r || !host) return fail(NADIA_EINVAL, "null argument");
-  if (!r->beam) {
-    int rc = alloc(&r->beam, r->npix * r->rsz);
-    if (rc) return rc;
-  }
-  HIPCHK(hipMemcpyAsync(r->beam, host, r->npix * r->rsz, hipMemcpyHostToDevice, S_(r)));
+  int rc;
+  if (!r->beam && (rc = alloc(&r->beam, r->npix * r->rsz))) return rc;
+  if ((rc = upload(r->ctx, r->beam, host, r->npix * r->rsz))) return rc;
   r->has_beam = 1;
   // re-mask sqrt(I) already on the device
   DISPATCH(r->prec, k_make_q<T>(nullptr, r->beam, 1, r->Q, r->npix, S_(r)));
   LAUNCHCHK();
-  HIPCHK(hipStreamSynchronize(S_(r)));
   return 0;
 }
 

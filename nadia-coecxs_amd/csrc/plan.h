@@ -60,8 +60,11 @@ constexpr bool size_supported(int n) {
          n == 192 || n == 384 || n == 768 || n == 1536 || n == 3072;
 }
 
-// Points per thread: row passes (contiguous transforms).
-constexpr int row_points(int n) { return (n % 3 == 0) ? (n < 24 ? n : 24) : (n < 8 ? n : 8); }
+// Points per thread: row passes (contiguous transforms).  3*2^k lengths take
+// 12 points (radix-4 and radix-3 stages): 24 points left the 384 and 768
+// passes at two waves per SIMD (114-150 registers, 55 KB of LDS per
+// workgroup) and 2-3x the time per pixel of the 2^k lengths.
+constexpr int row_points(int n) { return (n % 3 == 0) ? (n < 12 ? n : 12) : (n < 8 ? n : 8); }
 
 // Points per thread for column passes: 16 for 2^k >= 2048 (radix-16
 // stages, one LDS exchange fewer than the rows' radix-8 plan, and half the

@@ -125,8 +125,11 @@ class Server:
     """
 
     def __init__(self, backend, hbm_budget=200e9, max_active=None, tick=50, every=50, threshold=0.1,
-                 keep_results=False):
+                 keep_results=False, admit_per_tick=16):
         self.backend = backend
+        # asynchronous loop: requests admitted per pass, so that a burst of
+        # arrivals is uploaded while the device runs the jobs already in
+        self.admit_per_tick = admit_per_tick
         self.budget = float(hbm_budget)
         self.max_active = max_active
         self.tick = int(tick)
@@ -209,9 +212,11 @@ class Server:
         self.elapsed = clock() - t0
         return done
 
-    def _admit(self, waiting, active, used, t0, clock):
+    def _admit(self, waiting, active, used, t0, clock, limit=None):
         be = self.backend
-        while waiting and (self.max_active is None or len(active) < self.max_active):
+        n = 0
+        while waiting and (self.max_active is None or len(active) < self.max_active) and (limit is None or n < limit):
+            n += 1
             req = waiting[0]
             need = be.bytes_for(req)
             if need > self.budget:
@@ -252,13 +257,14 @@ class Server:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
-            used = self._admit(waiting, active, used, t0, clock)
-            if waiting and any(fin for _, fin in inflight):
+            before = len(active)
+            used = self._admit(waiting, active, used, t0, clock, self.admit_per_tick)
+            if waiting and len(active) - before < self.admit_per_tick and any(fin for _, fin in inflight):
                 # admission is blocked (memory or slots) and queued ticks
                 # hold finished jobs: free them first, then admit again
                 while inflight:
                     used -= self._collect(inflight.popleft(), done, t0, clock)
-                used = self._admit(waiting, active, used, t0, clock)
+                used = self._admit(waiting, active, used, t0, clock, self.admit_per_tick - (len(active) - before))
             self.peak_bytes = max(self.peak_bytes, used + self._held())
             self.peak_active = max(self.peak_active, len(active))
             if active:
