@@ -432,16 +432,49 @@ def leg_served(nd, synth, count, rate, budget):
         by.setdefault(c.n, []).append(c.latency * 1e3)
     s["p50_ms_by_size"] = {str(n): round(float(np.percentile(v, 50)), 3) for n, v in sorted(by.items())}
     s["p99_ms_by_size"] = {str(n): round(float(np.percentile(v, 99)), 3) for n, v in sorted(by.items())}
-    # capacity: the first 400 requests of the same mix arriving at once (the
-    # queue drains at the device's pace); sustained iterations/s and jobs/s
-    burst = [dataclasses.replace(r, arrival=0.0) for r in reqs[:400]]
+    s["capacity"] = leg_capacity(nd, serve, reqs, be, budget)
+    return s
+
+
+def leg_capacity(nd, serve, reqs, be, budget, count=400):
+    """Capacity: the first `count` requests of the served mix arriving at
+    once (the queue drains at the device's pace); sustained iterations/s,
+    jobs/s and pixel-iterations/s."""
+    burst = [dataclasses.replace(r, arrival=0.0) for r in reqs[:count]]
     srv = serve.Server(be, hbm_budget=budget)
     done = srv.run(burst)
     cap = serve.Server.summary(done, srv.elapsed)
-    s["capacity"] = {"requests": len(burst), "elapsed_s": round(srv.elapsed, 3),
-                     "aggregate_iter_s": round(cap["aggregate_iter_s"], 1),
-                     "jobs_per_s": round(len(burst) / srv.elapsed, 1), "peak_active": srv.peak_active}
-    return s
+    px = sum(c.iterations * c.n * c.n for c in done)
+    return {"requests": len(burst), "elapsed_s": round(srv.elapsed, 3),
+            "aggregate_iter_s": round(cap["aggregate_iter_s"], 1),
+            "pixel_iter_per_s": round(px / srv.elapsed, 1),
+            "jobs_per_s": round(len(burst) / srv.elapsed, 1), "peak_active": srv.peak_active,
+            "ticks": srv.ticks}
+
+
+def profile_leg(nd, synth, args):
+    """One leg alone (for profiler runs): its JSON on stdout."""
+    leg = args.profile
+    if leg == "batched":
+        out = leg_batched(nd, synth, args.batched_iters, args.graph_steps)
+    elif leg == "fresnel":
+        out = leg_fresnel(nd, synth)
+    elif leg == "phase_diverse":
+        out = leg_phase_diverse(nd, synth)
+    elif leg == "next_rows":
+        out = leg_next_rows(nd, synth, synth.planar_star(n=N))
+    elif leg == "capacity":
+        serve = importlib.import_module("nadia-coecxs_amd.serve")
+        reqs = serve.poisson_stream(count=args.serve_requests, rate=args.serve_rate)
+        sizes = sorted({r.n for r in reqs})
+        be = serve.DeviceBackend(nd, serve.input_pool(synth, sizes, per_size=2))
+        budget = args.serve_budget_gb * 1e9
+        serve.Server(be, hbm_budget=budget).run(
+            [serve.Request(-1 - i, 0.0, n, serve.HIO, 0.9, 20, False, 0) for i, n in enumerate(sizes)])
+        out = leg_capacity(nd, serve, reqs, be, budget)
+    else:
+        raise SystemExit("unknown leg %r" % leg)
+    print(json.dumps({leg: out}))
 
 
 def main():
@@ -468,6 +501,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=2000)
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--profile", default=None, metavar="LEG",
+                    choices=["batched", "fresnel", "phase_diverse", "next_rows", "capacity"],
+                    help="run one leg alone (for profiler runs) and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="instead of the benchmark, run K device copies of a 2048^2 complex64 field "
                          "(32 MiB read + 32 MiB written each) to calibrate profiler byte counters")
@@ -487,6 +523,9 @@ def main():
         nd.synchronize()
         print(json.dumps({"copy_calibration": args.copy_calibration, "bytes_read_per_copy": N * N * 8,
                           "bytes_written_per_copy": N * N * 8}))
+        return
+    if args.profile:
+        profile_leg(nd, synth, args)
         return
     t_gen = time.perf_counter()
     data = synth.planar_star(n=N)

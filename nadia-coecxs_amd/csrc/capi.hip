@@ -24,6 +24,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nadia_amd.h"
@@ -108,6 +109,26 @@ DeviceCtx* dev_ctx(int dev) {
   return &c;
 }
 
+// Host copy into the staging ring: large arrays are split over a few
+// threads (one thread's memcpy reaches about a third of the host's memory
+// bandwidth).
+void copy_host(char* dst, const void* src, size_t bytes) {
+  constexpr size_t kChunk = (size_t)4 << 20;
+  const int nt = bytes >= 4 * kChunk ? 4 : 1;
+  if (nt == 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t part = ((bytes / nt) + 255) & ~(size_t)255;
+  std::thread th[3];
+  for (int t = 1; t < nt; t++) {
+    const size_t a = t * part, b = std::min(bytes, (t + 1) * part);
+    th[t - 1] = std::thread([=] { std::memcpy(dst + a, (const char*)src + a, b - a); });
+  }
+  std::memcpy(dst, src, std::min(bytes, part));
+  for (int t = 1; t < nt; t++) th[t - 1].join();
+}
+
 // Queue a copy of `bytes` of pageable host memory to device memory `dst` on
 // the device's stream and return without waiting for the device (through
 // the staging ring; uploads larger than half of it copy directly and wait).
@@ -137,7 +158,7 @@ int upload(DeviceCtx* c, void* dst, const void* src, size_t bytes) {
   } else {
     HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   }
-  std::memcpy(st.base + off, src, bytes);
+  copy_host(st.base + off, src, bytes);
   HIPCHK(hipMemcpyAsync(dst, st.base + off, bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipEventRecord(ev, c->stream));
   st.uses.push_back({off, len, ev});
@@ -3315,6 +3336,10 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
     if ((rc = need_setup(r))) return rc;
   }
   if (b->cap < nj) {
+    // grow geometrically: freeing device memory waits for the device, so a
+    // batch that gains a few jobs per call (continuous batching) must not
+    // reallocate on every call
+    const int cap = std::max(nj, 2 * b->cap);
     if (b->gexec) {  // the graph addresses the old descriptor array
       HIPCHK(hipStreamSynchronize(c->stream));
       (void)hipGraphExecDestroy(b->gexec);
@@ -3322,8 +3347,8 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
     }
     if (b->djobs) (void)hipFree(b->djobs);
     b->djobs = nullptr;
-    if ((rc = alloc((void**)&b->djobs, sizeof(JobDesc) * nj))) return rc;
-    b->cap = nj;
+    if ((rc = alloc((void**)&b->djobs, sizeof(JobDesc) * cap))) return rc;
+    b->cap = cap;
   }
   std::vector<double> tmp;
   // copy the ring slots 0..m-1 of every job to errors rows base..base+m-1
@@ -3393,12 +3418,13 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       // through pinned staging: the copy is asynchronous, and the staging is
       // reused only after the previous copy from it has completed
       if (b->hcap < (int)host.size()) {
+        const int cap = std::max((int)host.size(), 2 * b->hcap);
         if (b->hev_pending) HIPCHK(hipEventSynchronize(b->hev));
         if (b->hjobs) (void)hipHostFree(b->hjobs);
         b->hjobs = nullptr;
         b->hcap = 0;
-        HIPCHK(hipHostMalloc((void**)&b->hjobs, sizeof(JobDesc) * host.size(), hipHostMallocDefault));
-        b->hcap = (int)host.size();
+        HIPCHK(hipHostMalloc((void**)&b->hjobs, sizeof(JobDesc) * cap, hipHostMallocDefault));
+        b->hcap = cap;
       } else if (b->hev_pending) {
         HIPCHK(hipEventSynchronize(b->hev));
       }

@@ -357,17 +357,18 @@ class DeviceBackend:
         d = self.inputs(req)
         cdt = np.complex128 if self.double else np.complex64
         key = (req.n, "beam_stop" in d)
+        x0 = np.ascontiguousarray(d["x0"], dtype=cdt)  # no copy when already in the serving precision
         if self.idle[key]:
             est, p = self.idle[key].pop()
             self.idle_bytes -= self._bytes[req.n]
-            est.set(d["x0"].astype(cdt))
+            est.set(x0)
             # back to a new object's algorithm state (ER, beta 0.9,
             # NADIA/src/BaseCDI.c++:33-66) so the request below selects its
             # algorithm exactly as on a fresh object
             p.set_relaxation_parameter(0.9)
             p.set_algorithm(ER)
         else:
-            est = nd.Complex2D(req.n, req.n, double=self.double, data=d["x0"].astype(cdt))
+            est = nd.Complex2D(req.n, req.n, double=self.double, data=x0)
             p = nd.PlanarCDI(est)
         p.set_support(d["support"])
         p.set_intensity(d["intensity"])
@@ -436,9 +437,18 @@ class DeviceBackend:
         return err, res
 
 
-def input_pool(synth, sizes, per_size=2, seed=7):
+def input_pool(synth, sizes, per_size=2, seed=7, double=False):
     """Inputs as the 2048^2 star-object workload with radius and support
     scaled by n/2048, `per_size` datasets per size (seeds seed + 10*k); a
-    request uses dataset req.seed % per_size of its size."""
-    pool = {n: [synth.planar_star(n=n, seed=seed + 10 * k) for k in range(per_size)] for n in sizes}
+    request uses dataset req.seed % per_size of its size.  The frames are
+    stored in the serving precision (float32 / complex64 unless double), as
+    a detector stream would deliver them, so admission only uploads."""
+    rdt, cdt = (np.float64, np.complex128) if double else (np.float32, np.complex64)
+
+    def frame(n, k):
+        d = synth.planar_star(n=n, seed=seed + 10 * k)
+        return {key: np.ascontiguousarray(v, dtype=cdt if key == "x0" else rdt)
+                for key, v in d.items() if key in ("intensity", "support", "x0", "beam_stop")}
+
+    pool = {n: [frame(n, k) for k in range(per_size)] for n in sizes}
     return lambda req: pool[req.n][req.seed % per_size]
