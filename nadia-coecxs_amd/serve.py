@@ -48,6 +48,15 @@ class Request:
     iterations: int
     shrinkwrap: bool    # shrinkwrap every `every` iterations
     seed: int
+    # configuration-file frames (frames.py): an algorithm schedule of
+    # (algorithm, iterations) blocks (algorithm is its first), and shrinkwrap
+    # after every sw_every iterations with a fixed width and threshold; the
+    # synthetic stream leaves them unset (one algorithm, shrinkwrap every
+    # `every` iterations with the width decreasing 3 -> 1.5)
+    schedule: tuple = None
+    sw_every: int = 0
+    sw_sigma: float = 1.5
+    sw_threshold: float = 0.1
 
 
 @dataclasses.dataclass
@@ -99,16 +108,51 @@ def percentile(values, q):
 
 
 class _Job:
-    __slots__ = ("req", "handle", "bytes", "done", "calls", "ncalls", "admitted", "pending")
+    """A running request: its iteration count and the events of its schedule
+    -- (after iteration count, kind, arguments) with kind "sw" (shrinkwrap:
+    width, threshold) or "alg" (switch algorithm) -- in order; a tick is cut
+    at the next event, and events at the same count run shrinkwrap first
+    (the reference tool shrinkwraps at the end of an iteration and switches
+    algorithm before the next block)."""
+    __slots__ = ("req", "handle", "bytes", "done", "admitted", "pending", "events")
 
-    def __init__(self, req, handle, nbytes, admitted, every):
+    def __init__(self, req, handle, nbytes, admitted, every, threshold):
         self.req = req
         self.handle = handle
         self.bytes = nbytes
         self.done = 0
-        self.calls = 0
-        self.ncalls = (req.iterations - 1) // every if req.shrinkwrap else 0
         self.admitted = admitted
+        ev = []
+        if req.sw_every > 0:
+            ev += [(k, 0, "sw", (req.sw_sigma, req.sw_threshold))
+                   for k in range(req.sw_every, req.iterations, req.sw_every)]
+        elif req.shrinkwrap:
+            ncalls = (req.iterations - 1) // every
+            ev += [((c + 1) * every, 0, "sw", (shrinkwrap_sigma(c, ncalls), threshold)) for c in range(ncalls)]
+        if req.schedule:
+            at = 0
+            for b in range(len(req.schedule) - 1):
+                at += req.schedule[b][1]
+                ev.append((at, 1, "alg", (req.schedule[b + 1][0],)))
+        ev.sort(key=lambda e: (e[0], e[1]))
+        self.events = collections.deque(e for e in ev if 0 < e[0] < req.iterations)
+
+    def count(self, tick):
+        """Iterations of the next tick: up to `tick`, cut at the next event."""
+        c = min(tick, self.req.iterations - self.done)
+        if self.events:
+            c = min(c, self.events[0][0] - self.done)
+        return c
+
+    def advance(self, c, be):
+        """Count c iterations as done and run the events that fall due."""
+        self.done += c
+        while self.events and self.events[0][0] == self.done:
+            _, _, kind, arg = self.events.popleft()
+            if kind == "sw":
+                be.shrinkwrap(self.handle, *arg)
+            else:
+                be.set_algorithm(self.handle, *arg)
 
 
 class Server:
@@ -171,7 +215,7 @@ class Server:
                     break
                 waiting.popleft()
                 h = be.start(req)
-                active.append(_Job(req, h, need, clock() - t0, self.every))
+                active.append(_Job(req, h, need, clock() - t0, self.every, self.threshold))
                 used += need
             self.peak_bytes = max(self.peak_bytes, used + self._held())
             self.peak_active = max(self.peak_active, len(active))
@@ -183,21 +227,12 @@ class Server:
                 continue
             # one tick: up to `tick` iterations per job, cut at its next
             # shrinkwrap point or its end
-            counts = []
-            for j in active:
-                left = j.req.iterations - j.done
-                c = min(self.tick, left)
-                if j.ncalls and j.calls < j.ncalls:
-                    c = min(c, (j.calls + 1) * self.every - j.done)
-                counts.append(c)
+            counts = [j.count(self.tick) for j in active]
             be.run([j.handle for j in active], counts)
             self.ticks += 1
             self.iterations += sum(counts)
             for j, c in zip(active, counts):
-                j.done += c
-                if j.ncalls and j.calls < j.ncalls and j.done == (j.calls + 1) * self.every:
-                    be.shrinkwrap(j.handle, shrinkwrap_sigma(j.calls, j.ncalls), self.threshold)
-                    j.calls += 1
+                j.advance(c, be)
             finished = [j for j in active if j.done >= j.req.iterations]
             if finished:
                 be.wait()
@@ -227,7 +262,7 @@ class Server:
                 break
             waiting.popleft()
             h = be.start(req)
-            active.append(_Job(req, h, need, clock() - t0, self.every))
+            active.append(_Job(req, h, need, clock() - t0, self.every, self.threshold))
             used += need
         return used
 
@@ -268,21 +303,12 @@ class Server:
             self.peak_bytes = max(self.peak_bytes, used + self._held())
             self.peak_active = max(self.peak_active, len(active))
             if active:
-                counts = []
-                for j in active:
-                    left = j.req.iterations - j.done
-                    c = min(self.tick, left)
-                    if j.ncalls and j.calls < j.ncalls:
-                        c = min(c, (j.calls + 1) * self.every - j.done)
-                    counts.append(c)
+                counts = [j.count(self.tick) for j in active]
                 be.run([j.handle for j in active], counts)
                 self.ticks += 1
                 self.iterations += sum(counts)
                 for j, c in zip(active, counts):
-                    j.done += c
-                    if j.ncalls and j.calls < j.ncalls and j.done == (j.calls + 1) * self.every:
-                        be.shrinkwrap(j.handle, shrinkwrap_sigma(j.calls, j.ncalls), self.threshold)
-                        j.calls += 1
+                    j.advance(c, be)
                 finished = [j for j in active if j.done >= j.req.iterations]
                 for j in finished:
                     j.pending = be.retire(j.handle, self.keep)
@@ -357,11 +383,15 @@ class DeviceBackend:
         d = self.inputs(req)
         cdt = np.complex128 if self.double else np.complex64
         key = (req.n, "beam_stop" in d)
-        x0 = np.ascontiguousarray(d["x0"], dtype=cdt)  # no copy when already in the serving precision
+        # no copy when already in the serving precision; without a starting
+        # point the estimate comes from initialise_estimate(seed), as in the
+        # reference tool (NADIA/tools/CDI_reconstruction.c)
+        x0 = np.ascontiguousarray(d["x0"], dtype=cdt) if "x0" in d else None
         if self.idle[key]:
             est, p = self.idle[key].pop()
             self.idle_bytes -= self._bytes[req.n]
-            est.set(x0)
+            if x0 is not None:
+                est.set(x0)
             # back to a new object's algorithm state (ER, beta 0.9,
             # NADIA/src/BaseCDI.c++:33-66) so the request below selects its
             # algorithm exactly as on a fresh object
@@ -374,6 +404,8 @@ class DeviceBackend:
         p.set_intensity(d["intensity"])
         if key[1]:
             p.set_beam_stop(d["beam_stop"])
+        if x0 is None:
+            p.initialise_estimate(req.seed)
         # set_relaxation_parameter before set_algorithm: selecting the
         # algorithm a job already has does nothing (NADIA/src/BaseCDI.c++:194-195)
         p.set_relaxation_parameter(req.beta)
@@ -388,6 +420,9 @@ class DeviceBackend:
 
     def shrinkwrap(self, handle, sigma, threshold):
         handle[2].apply_shrinkwrap(sigma, threshold)
+
+    def set_algorithm(self, handle, alg):
+        handle[2].set_algorithm(alg)
 
     def wait(self):
         self.nd.synchronize()

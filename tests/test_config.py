@@ -44,6 +44,7 @@ def probe(tmp_path_factory):
         out = subprocess.run([str(exe), str(path), *queries], capture_output=True, text=True, timeout=60)
         assert out.returncode == 0, out.stderr
         return out.stdout.splitlines()
+    run.path = cfg
     return run
 
 
@@ -94,3 +95,38 @@ def test_unreadable_file(probe, tmp_path):
     lines = probe("status", path=tmp_path / "absent.config")
     assert lines[0] == "Could not open the file " + str(tmp_path / "absent.config")
     assert lines[-1] == "status 0"
+
+
+def test_python_config_matches_cpp(probe, capsys):
+    """The request loop's front end (nadia-coecxs_amd/frames.py) reads
+    configuration files with a Python Config: every answer, message and
+    status equals the C++ Config's on the same file."""
+    import importlib
+
+    frames = importlib.import_module("nadia-coecxs_amd.frames")
+    queries = ["s:data_file_name", "i:pixels_x", "i:pixels_y", "i:indented_key", "s:output_file_name_prefix",
+               "d:beam_wavelength", "d:shrinkwrap_threshold", "s:dup", "i:int_with_text", "S:algorithm",
+               "I:iterations", "s:starting_point_file_name", "i:commented_out", "S:empty_list", "I:empty_list",
+               "d:no_separator_line"]
+    cpp = probe(*queries, "status")
+    capsys.readouterr()
+    c = frames.Config(str(probe.path))
+    lines = []
+    for q in queries:
+        kind, key = q[0], q[2:]
+        if kind == "s":
+            r = " [%s]" % c.get_string(key)
+        elif kind == "d":
+            r = " %.17g" % c.get_double(key)
+        elif kind == "i":
+            r = " %d" % c.get_int(key)
+        elif kind == "S":
+            v = c.get_string_list(key)
+            r = " null" if v is None else "".join(" [%s]" % w for w in v)
+        else:
+            v = c.get_int_list(key)
+            r = " null" if v is None else "".join(" %d" % w for w in v)
+        lines += capsys.readouterr().out.splitlines()
+        lines.append("= %s %s ->%s" % (kind, key, r))
+    lines.append("status %d" % c.status)
+    assert lines == cpp

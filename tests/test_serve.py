@@ -34,6 +34,7 @@ class FakeBackend:
         self.peak = 0
         self.started = []
         self.shrinks = {}
+        self.switches = {}
         self.counts_seen = []
 
     def bytes_for(self, req):
@@ -58,6 +59,9 @@ class FakeBackend:
 
     def shrinkwrap(self, h, sigma, thr):
         self.shrinks[h["req"].rid].append((h["done"], sigma, thr))
+
+    def set_algorithm(self, h, alg):
+        self.switches.setdefault(h["req"].rid, []).append((h["done"], alg))
 
     def wait(self):
         pass
@@ -219,3 +223,21 @@ def test_async_loop_matches_sync_schedule():
     assert {c.rid: c.iterations for c in d1} == {c.rid: c.iterations for c in d2}
     assert s2.elapsed <= s1.elapsed + 1e-9
     assert s2.peak_bytes <= 6 * 1024 * 1024 * 100
+
+
+@BACKENDS
+def test_schedule_requests_switch_algorithms_and_shrinkwrap(backend):
+    """Configuration-file frames: algorithm blocks switch at their cumulative
+    counts, shrinkwrap follows every sw_every-th iteration with the frame's
+    own width and threshold (never after the last), shrinkwrap before the
+    switch when both fall on one count."""
+    reqs = [serve.Request(0, 0.0, 256, serve.HIO, 0.9, 400, True, 0, schedule=((serve.HIO, 250), (serve.ER, 150)),
+                          sw_every=50, sw_sigma=2.0, sw_threshold=0.2),
+            serve.Request(1, 0.0, 512, serve.ER, 0.9, 90, False, 1, schedule=((serve.ER, 30), (serve.RAAR, 40),
+                                                                             (serve.HIO, 20)))]
+    srv, be, done = run(reqs, backend, hbm_budget=1e12, tick=64)
+    assert be.switches[0] == [(250, serve.ER)]
+    assert [s[0] for s in be.shrinks[0]] == list(range(50, 400, 50))
+    assert all(s[1:] == (2.0, 0.2) for s in be.shrinks[0])
+    assert be.switches[1] == [(30, serve.RAAR), (70, serve.HIO)] and be.shrinks[1] == []
+    assert {c.rid: c.iterations for c in done} == {0: 400, 1: 90}
