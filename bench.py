@@ -554,6 +554,20 @@ def main():
     # launch (the single-job rate is reported beside it)
     batch_jobs = 2
     value = dual
+    # eight jobs: 8 x 96 MB of hot state, well beyond the 256 MB Infinity
+    # Cache that holds one or two jobs -- the rate when every pass streams
+    # from HBM
+    jobs8 = [setup(nd, data, False, nd.PATH_FUSED) for _ in range(8)]
+    batch = nd.Batch([q for _, q in jobs8])
+    batch.iterate_async(max(2, args.warmup // 4))
+    nd.synchronize()
+    n8 = max(10, args.steps // 4)
+    t0 = time.perf_counter()
+    batch.iterate_async(n8)
+    nd.synchronize()
+    batch8 = 8 * n8 / (time.perf_counter() - t0)
+    print("bench: fp32 fused, batch of 8 jobs: %.1f job-iter/s" % batch8, file=sys.stderr, flush=True)
+    del batch, jobs8
     # per-pass device time with HIP events on the library stream (one job)
     p.set_timing(True)
     p.iterate_async(min(args.steps, 200))
@@ -693,6 +707,8 @@ def main():
                                "Poisson peak 1e8, 16x16 beam stop, 900x900 support, FP32 bulk + FP64 error sums",
                    "nx": N, "ny": N, "algorithm": "HIO", "beta": 0.9, "batch_jobs": batch_jobs},
         "fp32_single_job_iter_s": round(single, 2), "fp32_batch2_job_iter_s": round(dual, 2),
+        "fp32_batch8_job_iter_s": round(batch8, 2),
+        "fp32_batch8_moved_GBs": round(batch8 * (pass_bytes(False)["col"] + pass_bytes(False)["row"]) * N * N / 1e9, 1),
         "fp64_plain_iter_s": round(plain64, 2), "fp64_fused_iter_s": round(fused64, 2),
         "fp32_vs_fp64_plain": round(value / plain64, 2),
         "final_error_fp32": err32,
