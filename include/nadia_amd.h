@@ -76,7 +76,14 @@ const char* nadia_last_error(void);
 int nadia_version(void);
 int nadia_device_count(int* count);
 int nadia_set_device(int device);
+/* 1 if n is a valid axis length: 1..4096 (FFTW plans any length,
+ * NADIA/src/Complex_2D.c++:472-477).  Reconstructions also need even sizes
+ * (Complex_2D::invert assumes them, NADIA/src/Complex_2D.c++:318-331). */
 int nadia_size_supported(int n);
+/* 1 if n has compiled single-launch passes (2^k 8..4096, 3*2^k 24..3072);
+ * other lengths run transforms of any length and the unfused iteration.
+ * No counterpart in the reference: it tells callers which sizes are fast. */
+int nadia_size_native(int n);
 
 /* ---- Complex_2D (NADIA/include/Complex_2D.h:35-515, src/Complex_2D.c++) ---- */
 

@@ -53,6 +53,7 @@ def _sig(lib):
         "nadia_device_count": (I, [IP]),
         "nadia_set_device": (I, [I]),
         "nadia_size_supported": (I, [I]),
+        "nadia_size_native": (I, [I]),
         "nadia_synchronize": (I, []),
         "nadia_event_create": (I, [PP]),
         "nadia_event_destroy": (I, [P]),

@@ -76,6 +76,19 @@ struct Staging {
   std::vector<hipEvent_t> spare;   // completed events for reuse
 };
 
+// Device tables of a transform of any length (kern_any.inc): the stage list
+// and twiddles of the len-point plan and, for Bluestein, the chirp and the
+// kernel transform.  One allocation, kept until the process ends.
+struct AnyPlan {
+  int n = 0, len = 0, nst = 0;
+  int radix[kAnyMaxStages] = {};
+  bool blue = false;
+  const int* stages = nullptr;
+  const void* tw = nullptr;
+  const void* chirp = nullptr;
+  const void* kern = nullptr;
+};
+
 struct DeviceCtx {
   hipStream_t stream = nullptr;
   Staging stage;
@@ -84,6 +97,7 @@ struct DeviceCtx {
   // the first allocation-free call on the device)
   void* scratch = nullptr;
   std::map<std::pair<int, int>, void*> twiddles;  // (n, precision) -> table
+  std::map<std::pair<int, int>, struct AnyPlan> anyplans;  // (n, precision) -> any-length plan
   // side streams (default priority) and events that fork independent groups
   // of a batch off `stream` while a graph is captured
   std::vector<hipStream_t> side;
@@ -216,6 +230,206 @@ void* twiddles(DeviceCtx* c, int n, int prec, bool row) {
     return ins.first->second;
   }
   return dptr;
+}
+
+// ---- transforms of any length ---------------------------------------------
+// Lengths without compiled single-launch passes run kern_any.inc's tile
+// kernel per axis.  A length whose prime factors are all <= the precision's
+// largest radix (13 in FP32, 7 in FP64) gets a direct mixed-radix plan
+// (radix 8 first, then 4, 2, 3, 5, 7, 11, 13); any other length Bluestein's:
+// X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_j = exp(-i pi j^2 / n), the
+// convolution computed with a power-of-two transform of length >= 2n - 1.
+constexpr int kAnyMaxLen = 4096;
+
+bool any_factor(int n, int maxr, std::vector<int>& rad) {
+  rad.clear();
+  int m = n;
+  for (int r : {8, 4, 2}) while (m % r == 0) { rad.push_back(r); m /= r; }
+  for (int r : {3, 5, 7, 11, 13}) {
+    if (r > maxr) break;
+    while (m % r == 0) { rad.push_back(r); m /= r; }
+  }
+  return m == 1 && (int)rad.size() <= kAnyMaxStages;
+}
+
+// DFT (sign -1) of a power-of-two length sequence in long double, in place
+void host_fft_pow2(std::vector<long double>& re, std::vector<long double>& im) {
+  const size_t n = re.size();
+  for (size_t i = 1, j = 0; i < n; ++i) {  // bit reversal
+    size_t bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) { std::swap(re[i], re[j]); std::swap(im[i], im[j]); }
+  }
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (size_t len = 2; len <= n; len <<= 1)
+    for (size_t k = 0; k < len / 2; ++k) {
+      const long double a = -twopi * (long double)k / (long double)len;
+      const long double wr = cosl(a), wi = sinl(a);
+      for (size_t i = 0; i < n; i += len) {
+        const size_t u = i + k, v = i + k + len / 2;
+        const long double xr = re[v] * wr - im[v] * wi, xi = re[v] * wi + im[v] * wr;
+        re[v] = re[u] - xr; im[v] = im[u] - xi;
+        re[u] += xr; im[u] += xi;
+      }
+    }
+}
+
+const AnyPlan* any_plan(DeviceCtx* c, int n, int prec) {
+  const auto key = std::make_pair(n, prec);
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = c->anyplans.find(key);
+    if (it != c->anyplans.end()) return &it->second;
+  }
+  if (n < 2 || n > kAnyMaxLen) return nullptr;
+  AnyPlan p;
+  p.n = n;
+  std::vector<int> rad;
+  const int maxr = any_max_radix(prec == NADIA_FP64 ? 8 : 4);
+  p.blue = !any_factor(n, maxr, rad);
+  p.len = n;
+  if (p.blue) {
+    p.len = 1;
+    while (p.len < 2 * n - 1) p.len <<= 1;
+    any_factor(p.len, maxr, rad);
+  }
+  p.nst = (int)rad.size();
+  // stage list and twiddles W_{L r}^{k m} of the len-point plan
+  std::vector<int> st(3 * kAnyMaxStages, 0);
+  std::vector<long double> wr, wi;
+  const long double twopi = 6.283185307179586476925286766559005768L, pi = twopi / 2;
+  int L = 1;
+  for (int s = 0; s < p.nst; ++s) {
+    const int r = rad[s];
+    p.radix[s] = r;
+    st[3 * s] = r;
+    st[3 * s + 1] = L;
+    st[3 * s + 2] = (int)wr.size();
+    for (int k = 0; k < L; ++k)
+      for (int m = 1; m < r; ++m) {
+        const long long t = ((long long)k * m) % ((long long)L * r);
+        const long double a = -twopi * (long double)t / (long double)(L * r);
+        wr.push_back(cosl(a));
+        wi.push_back(sinl(a));
+      }
+    L *= r;
+  }
+  const size_t ntw = wr.size();
+  std::vector<long double> cr, ci, kr, ki;
+  if (p.blue) {
+    cr.resize(n); ci.resize(n);
+    for (int j = 0; j < n; ++j) {  // exp(-i pi j^2 / n), j^2 reduced mod 2n exactly
+      const long long q = ((long long)j * j) % (2LL * n);
+      const long double a = -pi * (long double)q / (long double)n;
+      cr[j] = cosl(a);
+      ci[j] = sinl(a);
+    }
+    kr.assign(p.len, 0.0L); ki.assign(p.len, 0.0L);
+    for (int j = 0; j < n; ++j) {
+      kr[j] = cr[j]; ki[j] = -ci[j];
+      if (j > 0) { kr[p.len - j] = cr[j]; ki[p.len - j] = -ci[j]; }
+    }
+    host_fft_pow2(kr, ki);
+    for (int j = 0; j < p.len; ++j) { kr[j] /= p.len; ki[j] /= p.len; }
+  }
+  const size_t esz = prec == NADIA_FP64 ? 16 : 8;
+  const size_t o_tw = 256, o_ch = o_tw + ((ntw * esz + 255) & ~(size_t)255),
+               o_k = o_ch + ((cr.size() * esz + 255) & ~(size_t)255), total = o_k + kr.size() * esz;
+  std::vector<unsigned char> h(total, 0);
+  std::memcpy(h.data(), st.data(), st.size() * sizeof(int));
+  auto put = [&](size_t off, const std::vector<long double>& re, const std::vector<long double>& im) {
+    for (size_t i = 0; i < re.size(); ++i) {
+      if (prec == NADIA_FP64) {
+        const double v[2] = {(double)re[i], (double)im[i]};
+        std::memcpy(h.data() + off + i * esz, v, sizeof(v));
+      } else {
+        const float v[2] = {(float)re[i], (float)im[i]};
+        std::memcpy(h.data() + off + i * esz, v, sizeof(v));
+      }
+    }
+  };
+  put(o_tw, wr, wi);
+  put(o_ch, cr, ci);
+  put(o_k, kr, ki);
+  void* d = nullptr;
+  if (hipMalloc(&d, total) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), total, hipMemcpyHostToDevice) != hipSuccess) {  // never cache an unfilled table
+    (void)hipFree(d);
+    return nullptr;
+  }
+  p.stages = (const int*)d;
+  p.tw = (const char*)d + o_tw;
+  p.chirp = p.blue ? (const char*)d + o_ch : nullptr;
+  p.kern = p.blue ? (const char*)d + o_k : nullptr;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto ins = c->anyplans.emplace(key, p);
+  if (!ins.second) (void)hipFree(d);  // another thread built it meanwhile
+  return &ins.first->second;
+}
+
+// Launch shape of kern_any.inc's kernel for the sequences of one axis:
+// sequences per workgroup, LDS stride, LDS class and workgroup size such
+// that every stage's butterflies fit the threads' registers.
+struct AnyShape {
+  int cb, stride, lds_class, threads;
+};
+
+bool any_shape(const AnyPlan& p, int nseq, bool rows, int prec, AnyShape& sh) {
+  const int esz = prec == NADIA_FP64 ? 16 : 8;
+  const int V = any_vals(esz / 2);
+  int cb = rows ? std::max(1, 1024 / p.len) : 64 / esz;  // rows: ~1k elements; columns: 64-byte segments
+  cb = std::max(1, std::min(cb, nseq));
+  for (; cb >= 1; cb = cb > 1 ? cb / 2 : 0) {
+    const int stride = (rows || cb == 1) ? p.len : p.len + 1;
+    const size_t bytes = (size_t)cb * stride * esz;
+    int cls = -1;
+    for (int k = 0; k < 3 && cls < 0; ++k)
+      if (bytes <= (size_t)kAnyLds[k]) cls = k;
+    if (cls < 0) continue;
+    int t = 64;
+    for (int s = 0; s < p.nst; ++s) {
+      const int r = p.radix[s], nb = std::max(1, V / r);
+      const int need = ((p.len / r) * cb + nb - 1) / nb;
+      t = std::max(t, need);
+    }
+    t = (t + 63) / 64 * 64;
+    if (t > 1024) continue;
+    sh = AnyShape{cb, stride, cls, t};
+    return true;
+  }
+  return false;
+}
+
+// the transforms of length n along one axis of an nx x ny field
+template <typename T>
+int any_axis(DeviceCtx* c, void* buf, int nx, int ny, int dir, bool rows, hipStream_t st) {
+  const int prec = sizeof(T) == 8 ? NADIA_FP64 : NADIA_FP32;
+  const int n = rows ? ny : nx;
+  if (n == 1) return 0;
+  const AnyPlan* p = any_plan(c, n, prec);
+  if (!p) return fail(NADIA_EHIP, "transform tables for length %d: allocation failed", n);
+  AnyArgs a{};
+  a.buf = buf;
+  a.rows = rows ? 1 : 0;
+  a.nseq = rows ? nx : ny;
+  a.es = rows ? 1 : ny;
+  a.ss = rows ? ny : 1;
+  a.n = n;
+  a.len = p->len;
+  a.dir = dir;
+  a.nst = p->nst;
+  a.stages = p->stages;
+  a.tw = p->tw;
+  a.chirp = p->chirp;
+  a.kern = p->kern;
+  AnyShape sh{};
+  if (!any_shape(*p, a.nseq, rows, prec, sh)) return fail(NADIA_ESIZE, "no launch shape for length %d", n);
+  a.cb = sh.cb;
+  a.stride = sh.stride;
+  launch_any<T>(a, sh.lds_class, sh.threads, st);
+  LAUNCHCHK();
+  return 0;
 }
 
 // Both tables of an nx x ny transform, built before any stream capture (the
@@ -411,6 +625,11 @@ hipStream_t S_(const nadia_recon* r) { return r->ctx->stream; }
 template <typename T>
 int fft2d_T(DeviceCtx* c, void* buf, int nx, int ny, int dir) {
   const int prec = sizeof(T) == 8 ? NADIA_FP64 : NADIA_FP32;
+  if (!size_native(nx) || !size_native(ny)) {  // lengths without compiled passes
+    int rc;
+    if ((rc = any_axis<T>(c, buf, nx, ny, dir, true, c->stream))) return rc;
+    return any_axis<T>(c, buf, nx, ny, dir, false, c->stream);
+  }
   void* twy = twiddles(c, ny, prec, true);
   void* twx = twiddles(c, nx, prec, false);
   if (!twy || !twx) return fail(NADIA_EHIP, "twiddle table allocation failed");
@@ -450,8 +669,8 @@ int fft2d(DeviceCtx* c, void* buf, int nx, int ny, int prec, int dir) {
   } while (0)
 
 int check_dims(int nx, int ny) {
-  if (!size_supported(nx) || !size_supported(ny))
-    return fail(NADIA_ESIZE, "unsupported size %dx%d (supported per axis: 2^k 8..4096, 3*2^k 24..3072)", nx, ny);
+  if (nx < 1 || ny < 1 || nx > kAnyMaxLen || ny > kAnyMaxLen)
+    return fail(NADIA_ESIZE, "unsupported size %dx%d (supported per axis: 1..%d)", nx, ny, kAnyMaxLen);
   return 0;
 }
 
@@ -641,12 +860,16 @@ bool has_projection(const nadia_recon* r) {
 // a transmission constraint needs whole-field reductions between the
 // projection and the update, so a constrained job does not share the fused
 // launches (ER takes fused_tc_ok's path, the rest the unfused path)
-bool fused_ok(const nadia_recon* r) { return r->path == NADIA_PATH_FUSED && has_projection(r) && !r->tc; }
+// (lengths without compiled passes always take the unfused path)
+bool native_dims(const nadia_recon* r) { return size_native(r->nx) && size_native(r->ny); }
+bool fused_ok(const nadia_recon* r) {
+  return r->path == NADIA_PATH_FUSED && native_dims(r) && has_projection(r) && !r->tc;
+}
 
 // ER with a transmission constraint: the fused passes give support(P_F x),
 // the constraint then runs on the new estimate between iterations
 bool fused_tc_ok(const nadia_recon* r) {
-  return r->path == NADIA_PATH_FUSED && has_projection(r) && r->tc && r->algorithm == NADIA_ER &&
+  return r->path == NADIA_PATH_FUSED && native_dims(r) && has_projection(r) && r->tc && r->algorithm == NADIA_ER &&
          r->kind != NADIA_FRESNEL_WF;
 }
 
@@ -1189,7 +1412,9 @@ int nadia_set_device(int device) {
   return 0;
 }
 
-int nadia_size_supported(int n) { return size_supported(n) ? 1 : 0; }
+int nadia_size_supported(int n) { return n >= 1 && n <= kAnyMaxLen ? 1 : 0; }
+
+int nadia_size_native(int n) { return size_native(n) ? 1 : 0; }
 
 int nadia_synchronize(void) {
   DeviceCtx* c = dev_ctx(current_device());

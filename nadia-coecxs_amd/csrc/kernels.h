@@ -109,6 +109,32 @@ struct ConvArgs {
   double* partials; // one max per block
 };
 
+// 1-D transforms of any length (kern_any.inc): the sequences of a field
+// along one axis, cb whole sequences per workgroup held in LDS.
+constexpr int kAnyMaxStages = 16;
+constexpr int kAnyLds[3] = {16 * 1024, 64 * 1024, 128 * 1024};  // LDS classes (bytes per workgroup)
+// complex values a thread holds per stage (registers): 16 float2 / 8 double2;
+// the largest prime radix of a direct plan: 13 in FP32, 7 in FP64 (larger
+// ones spill registers), beyond that Bluestein
+constexpr int any_max_radix(int real_bytes) { return real_bytes == 4 ? 13 : 7; }
+constexpr int any_vals(int real_bytes) { return real_bytes == 4 ? 16 : 8; }
+struct AnyArgs {
+  void* buf;            // field, transformed in place
+  long long es, ss;     // element stride and sequence stride (complex elements)
+  int nseq;             // sequences (rows or columns)
+  int rows;             // 1: es == 1 (load along a sequence), 0: ss == 1 (across sequences)
+  int n;                // transform length
+  int len;              // LDS sequence length: n, or Bluestein's power-of-two m >= 2n - 1
+  int cb, stride;       // sequences per workgroup, LDS elements between sequences
+  int dir;              // -1 forward, +1 backward (unnormalised)
+  int nst;              // stages of the len-point plan
+  const int* stages;    // device: (radix, span, twiddle offset) per stage
+  const void* tw;       // device: twiddles of the len-point plan, sign -1
+  const void* chirp;    // Bluestein: exp(-i pi k^2 / n), k < n; null for a direct plan
+  const void* kern;     // Bluestein: DFT_len of the conjugate chirp kernel, times 1/len
+};
+template <typename T> void launch_any(const AnyArgs& a, int lds_class, int threads, hipStream_t st);
+
 // v selects the tile variant (plan.h: col_cfg / row_cfg); all variants of a
 // length compute identical results
 template <typename T> void launch_row(const RowArgs& a, int njobs, int v, hipStream_t st);

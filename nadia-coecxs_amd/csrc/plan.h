@@ -53,8 +53,10 @@ constexpr PlanDesc make_plan(int n, int p) {
 
 constexpr int plan_twiddles(const PlanDesc& d) { return d.nst > 0 ? d.tw_off[d.nst] : 0; }
 
-// Lengths with compiled kernels: 2^k (8..4096) and 3*2^k (24..3072).
-constexpr bool size_supported(int n) {
+// Lengths with compiled single-launch passes: 2^k (8..4096) and 3*2^k
+// (24..3072).  Other lengths take the any-length transforms (kern_any.inc)
+// and the unfused iteration.
+constexpr bool size_native(int n) {
   return n == 8 || n == 16 || n == 32 || n == 64 || n == 128 || n == 256 || n == 512 ||
          n == 1024 || n == 2048 || n == 4096 || n == 24 || n == 48 || n == 96 ||
          n == 192 || n == 384 || n == 768 || n == 1536 || n == 3072;

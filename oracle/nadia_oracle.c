@@ -758,11 +758,16 @@ int ocdi_iterate(ocdi *o) {
   return 1;
 }
 
-/* BaseCDI::apply_threshold: NADIA/src/BaseCDI.c++:477-492 (loops j < nx) */
+/* BaseCDI::apply_threshold: NADIA/src/BaseCDI.c++:477-492 (loops j < nx, so
+ * for nx < ny the columns j >= nx keep their values).  For nx > ny the
+ * reference's loop runs past the end of each row and, on the last row, past
+ * the array (undefined behaviour); here each pixel is then thresholded once,
+ * as the device kernel does. */
 static void apply_threshold(ocdi *o, real *a, double threshold) {
   double max = r_max(a, NPIX(o));
+  const int jn = o->nx < o->ny ? o->nx : o->ny;
   for (int i = 0; i < o->nx; i++)
-    for (int j = 0; j < o->nx; j++) {
+    for (int j = 0; j < jn; j++) {
       size_t p = (size_t)i * o->ny + j;
       if (a[p] < threshold * max) a[p] = 0.0;
       else a[p] = 1.0;

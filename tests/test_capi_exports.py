@@ -24,7 +24,9 @@ def test_header_symbols_exported(nd):
 def test_host_only_entry_points(nd):
     lib = nd.library()
     assert lib.nadia_version() >= 100
-    for n, ok in ((2048, 1), (384, 1), (768, 1), (24, 1), (100, 0), (6, 0), (8192, 0)):
+    for n, ok, native in ((2048, 1, 1), (384, 1, 1), (768, 1, 1), (24, 1, 1), (100, 1, 0), (6, 1, 0), (1, 1, 0),
+                          (4096, 1, 1), (4097, 0, 0), (8192, 0, 0), (0, 0, 0)):
         assert lib.nadia_size_supported(n) == ok
+        assert lib.nadia_size_native(n) == native
     assert nd.get_alg_from_name("RAAR") == nd.RAAR
     assert nd.get_alg_from_name("nope") == -1

@@ -18,7 +18,7 @@ def last_error(nd):
     return nd.library().nadia_last_error().decode()
 
 
-@pytest.mark.parametrize("shape", [(100, 100), (0, 64), (64, 8192), (6, 8), (48, 5000)])
+@pytest.mark.parametrize("shape", [(0, 64), (64, 8192), (48, 5000), (4097, 2), (8, -1)])
 def test_unsupported_sizes(nd, shape):
     h = ctypes.c_void_p()
     assert nd.library().nadia_field_create(shape[0], shape[1], nd.FP64, ctypes.byref(h)) == ESIZE
