@@ -402,6 +402,23 @@ def leg_next_rows(nd, synth, data2048):
         p.get_intensity_autocorrelation()
     out["autocorrelation_2048_fp32_ms"] = round((time.perf_counter() - t0) * 100, 3)
     del p, est
+    # sizes without compiled passes (any-length transforms, unfused
+    # iteration) beside the native 1024^2: 1000 = 8*5^3 (direct plan), 1030 =
+    # 2*5*103 (Bluestein, transforms of length 4096)
+    for n in (1024, 1000, 1030):
+        d = synth.planar_block(n=n, block=3 * n // 8, support=7 * n // 16)
+        est = nd.Complex2D(n, n, double=False, data=d["x0"].astype(np.complex64))
+        p = nd.PlanarCDI(est)
+        p.set_support(d["support"])
+        p.set_intensity(d["intensity"])
+        p.set_algorithm(nd.HIO)
+        p.iterate_async(10)
+        nd.synchronize()
+        t0 = time.perf_counter()
+        p.iterate_async(200)
+        nd.synchronize()
+        out["hio_%d_fp32_iter_s" % n] = round(200 / (time.perf_counter() - t0), 1)
+        del p, est
     return out
 
 

@@ -378,7 +378,11 @@ struct AnyShape {
 bool any_shape(const AnyPlan& p, int nseq, bool rows, int prec, AnyShape& sh) {
   const int esz = prec == NADIA_FP64 ? 16 : 8;
   const int V = any_vals(esz / 2);
-  int cb = rows ? std::max(1, 1024 / p.len) : 64 / esz;  // rows: ~1k elements; columns: 64-byte segments
+  // rows: about 1k elements per workgroup; columns: 64-byte row segments,
+  // down to 32 bytes while that leaves fewer workgroups than CUs (256)
+  int cb = rows ? std::max(1, 1024 / p.len) : 64 / esz;
+  if (!rows)
+    while (cb * esz > 32 && (nseq + cb - 1) / cb < 256) cb /= 2;
   cb = std::max(1, std::min(cb, nseq));
   for (; cb >= 1; cb = cb > 1 ? cb / 2 : 0) {
     const int stride = (rows || cb == 1) ? p.len : p.len + 1;

@@ -113,11 +113,11 @@ struct ConvArgs {
 // along one axis, cb whole sequences per workgroup held in LDS.
 constexpr int kAnyMaxStages = 16;
 constexpr int kAnyLds[3] = {16 * 1024, 64 * 1024, 128 * 1024};  // LDS classes (bytes per workgroup)
-// complex values a thread holds per stage (registers): 16 float2 / 8 double2;
+// complex values a thread holds per stage (registers, at least one butterfly): 8;
 // the largest prime radix of a direct plan: 13 in FP32, 7 in FP64 (larger
 // ones spill registers), beyond that Bluestein
 constexpr int any_max_radix(int real_bytes) { return real_bytes == 4 ? 13 : 7; }
-constexpr int any_vals(int real_bytes) { return real_bytes == 4 ? 16 : 8; }
+constexpr int any_vals(int real_bytes) { return real_bytes == 4 ? 8 : 8; }
 struct AnyArgs {
   void* buf;            // field, transformed in place
   long long es, ss;     // element stride and sequence stride (complex elements)
