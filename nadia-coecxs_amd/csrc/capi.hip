@@ -1319,6 +1319,92 @@ int iterate_fused_tc(nadia_recon* r, int n) {
   return 0;
 }
 
+// ---- fused iterations at sizes without compiled passes ----------------------
+// The column and row passes of the fused iteration on kern_any.inc's tiles:
+// W = row DFT of X between iterations, a column pass (forward DFT,
+// modulus projection, inverse) and a row pass (inverse DFT, update of X,
+// forward DFT), one job per launch.  PlanarCDI and FresnelCDI with
+// algorithms of one work field; the rest take the unfused path.
+bool any_fused_ok(const nadia_recon* r) {
+  return r->path == NADIA_PATH_FUSED && !native_dims(r) && has_projection(r) && !r->tc && !two_fields(r) &&
+         (r->kind == NADIA_PLANAR || r->kind == NADIA_FRESNEL);
+}
+
+// one any-length pass of reconstruction r: mode, error slot, column-pass
+// workgroups (for the row pass's error); *nblk = workgroups launched
+template <typename T>
+int any_pass(nadia_recon* r, int mode, int iter, int fin_n, int* nblk) {
+  const int prec = sizeof(T) == 8 ? NADIA_FP64 : NADIA_FP32;
+  const bool rows = mode == ANY_ROW_UPDATE;
+  const int n = rows ? r->ny : r->nx;
+  const AnyPlan* p = any_plan(r->ctx, n, prec);
+  if (!p) return fail(NADIA_EHIP, "transform tables for length %d: allocation failed", n);
+  AnyArgs a{};
+  a.mode = mode;
+  a.jd = r->djob;
+  a.nx = r->nx;
+  a.ny = r->ny;
+  a.sf = 1.0 / std::sqrt((double)r->nx * (double)r->ny);
+  a.iter = iter;
+  a.fin_n = fin_n;
+  a.rows = rows ? 1 : 0;
+  a.nseq = rows ? r->nx : r->ny;
+  a.es = rows ? 1 : r->ny;
+  a.ss = rows ? r->ny : 1;
+  a.n = n;
+  a.len = p->len;
+  a.nst = p->nst;
+  a.stages = p->stages;
+  a.tw = p->tw;
+  a.chirp = p->chirp;
+  a.kern = p->kern;
+  AnyShape sh{};
+  if (!any_shape(*p, a.nseq, rows, prec, sh)) return fail(NADIA_ESIZE, "no launch shape for length %d", n);
+  a.cb = sh.cb;
+  a.stride = sh.stride;
+  launch_any<T>(a, sh.lds_class, sh.threads, S_(r));
+  LAUNCHCHK();
+  if (nblk) *nblk = (a.nseq + a.cb - 1) / a.cb;
+  return 0;
+}
+
+// W current (rebuilt from X when X changed) and the job descriptor uploaded
+template <typename T>
+int any_ready(nadia_recon* r) {
+  int rc;
+  if (w_stale(r)) {
+    HIPCHK(hipMemcpyAsync(r->W, r->est->d, r->npix * r->esz, hipMemcpyDeviceToDevice, S_(r)));
+    if ((rc = any_axis<T>(r->ctx, r->W, r->nx, r->ny, -1, true, S_(r)))) return rc;
+    r->w_valid = true;
+    r->w_version = r->est->version;
+  }
+  const JobDesc hj = make_jobdesc(r, r->est->d, r->W, r->errs);
+  if (r->djob_set && std::memcmp(&hj, &r->djob_host, sizeof(JobDesc)) == 0) return 0;
+  r->djob_host = hj;
+  r->djob_set = true;
+  return upload_jobs(S_(r), r->djob, &hj, 1);
+}
+
+// one iteration, its error into ring slot k (after any_ready)
+template <typename T>
+int any_step(nadia_recon* r, int k) {
+  int rc, nb = 0;
+  if ((rc = any_pass<T>(r, r->kind == NADIA_FRESNEL ? ANY_COL_FRESNEL : ANY_COL_PLANAR, k, 0, &nb))) return rc;
+  if ((rc = any_pass<T>(r, ANY_ROW_UPDATE, k, nb, nullptr))) return rc;
+  if (wants_n_best(r) && (rc = update_n_best(r, r->errs + k, S_(r)))) return rc;
+  return 0;
+}
+
+template <typename T>
+int iterate_any(nadia_recon* r, int n) {
+  int rc;
+  if ((rc = any_ready<T>(r))) return rc;
+  for (int k = 0; k < n; ++k)
+    if ((rc = any_step<T>(r, k))) return rc;
+  fused_done(r, n);
+  return 0;
+}
+
 int fetch_error(nadia_recon* r, double* out) {
   if (r->h_err_valid) {
     *out = r->h_err_cache;
@@ -1854,6 +1940,9 @@ int nadia_recon_iterate(nadia_recon* r, int n, double* errors) {
       if (rc) return rc;
     } else if (fused_tc_ok(r)) {
       DISPATCH(r->prec, rc = iterate_fused_tc<T>(r, chunk));
+      if (rc) return rc;
+    } else if (any_fused_ok(r)) {
+      DISPATCH(r->prec, rc = iterate_any<T>(r, chunk));
       if (rc) return rc;
     } else {
       for (int k = 0; k < chunk; ++k) {
@@ -3715,17 +3804,22 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
     // jobs outside the shared launches: ER with a transmission constraint
     // takes its own fused iterations (as when it runs alone), the rest the
     // unfused path
-    for (int j : plain)
-      if (fused_tc_ok(b->jobs[j])) {
-        DISPATCH(b->jobs[j]->prec, rc = fused_ready<T>(b->jobs[j]));
-        if (rc) return rc;
-      }
+    for (int j : plain) {
+      if (fused_tc_ok(b->jobs[j])) DISPATCH(b->jobs[j]->prec, rc = fused_ready<T>(b->jobs[j]));
+      else if (any_fused_ok(b->jobs[j])) DISPATCH(b->jobs[j]->prec, rc = any_ready<T>(b->jobs[j]));
+      if (rc) return rc;
+    }
     for (int k = eager_from; k < chunk; ++k)
       for (int j : plain) {
         nadia_recon* r = b->jobs[j];
         if (rem[j] <= k) continue;
         if (fused_tc_ok(r)) {
           DISPATCH(r->prec, rc = fused_tc_step<T>(r, k - eager_from));
+          if (rc) return rc;
+          continue;
+        }
+        if (any_fused_ok(r)) {
+          DISPATCH(r->prec, rc = any_step<T>(r, k - eager_from));
           if (rc) return rc;
           continue;
         }
@@ -3744,7 +3838,7 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       if (rem[j] == 0) continue;
       nadia_recon* r = b->jobs[j];
       if (fused_ok(r)) fused_done(r, graph ? last_slot + 1 : rem[j]);
-      else if (fused_tc_ok(r)) fused_done(r, rem[j]);
+      else if (fused_tc_ok(r) || any_fused_ok(r)) fused_done(r, rem[j]);
       else r->est->version++;
     }
     done += chunk;

@@ -118,7 +118,19 @@ constexpr int kAnyLds[3] = {16 * 1024, 64 * 1024, 128 * 1024};  // LDS classes (
 // ones spill registers), beyond that Bluestein
 constexpr int any_max_radix(int real_bytes) { return real_bytes == 4 ? 13 : 7; }
 constexpr int any_vals(int real_bytes) { return real_bytes == 4 ? 8 : 8; }
+// ANY_FFT: the transforms of `buf` along one axis.  The fused modes work on
+// one job's W (JobDesc w), like the single-launch passes: ANY_COL_PLANAR /
+// ANY_COL_FRESNEL = forward column DFT -> modulus projection -> inverse
+// (error partials per workgroup); ANY_ROW_UPDATE = inverse row DFT -> the
+// algorithm's update of x (one work field) -> forward row DFT, and the
+// column pass's error published by workgroup 0.
+enum AnyMode { ANY_FFT = 0, ANY_COL_PLANAR = 1, ANY_COL_FRESNEL = 2, ANY_ROW_UPDATE = 3 };
 struct AnyArgs {
+  int mode;
+  const JobDesc* jd;    // fused modes: the job (device)
+  int nx, ny;           // field size (fused modes)
+  double sf;            // 1/sqrt(nx*ny)
+  int iter, fin_n;      // ANY_ROW_UPDATE: error ring slot, column-pass workgroups
   void* buf;            // field, transformed in place
   long long es, ss;     // element stride and sequence stride (complex elements)
   int nseq;             // sequences (rows or columns)

@@ -96,12 +96,16 @@ def planar_state(shape, seed=11):
 PLANAR_SHAPES = [(100, 126), (74, 130), (60, 22)]
 
 
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
 @pytest.mark.parametrize("shape", PLANAR_SHAPES, ids=lambda s: "%dx%d" % s)
-@pytest.mark.parametrize("alg", ["HIO", "RAAR", "ER"])
-def test_planar_any_size_matches_oracle(nd, oracle, shape, alg):
+@pytest.mark.parametrize("alg", ["HIO", "RAAR", "ER", "DM"])
+def test_planar_any_size_matches_oracle(nd, oracle, shape, alg, path):
+    """Fused: kern_any.inc's column and row passes (DM, with its P_F P_S
+    term, runs unfused either way); plain: the reference's data flow."""
     d = planar_state(shape)
     est = nd.Complex2D(*shape, data=d["x0"])
     g = nd.PlanarCDI(est)
+    g.set_path(path)
     xo = d["x0"].copy()
     o = oracle.OracleCDI(xo, oracle.PLANAR)
     a = nd.get_alg_from_name(alg)
@@ -116,8 +120,8 @@ def test_planar_any_size_matches_oracle(nd, oracle, shape, alg):
     eg = g.iterate(30)
     eo = o.iterate_n(30)
     assert (np.abs(eg - eo) / eo).max() <= 1e-9
-    # fused is a request: sizes without compiled passes run unfused, same results
-    g.set_path(0)
+    # switching paths mid-run continues from the same state
+    g.set_path(1 - path)
     eg = g.iterate(5)
     eo = o.iterate_n(5)
     assert (np.abs(eg - eo) / eo).max() <= 1e-9
@@ -167,6 +171,30 @@ def test_planar_any_size_shrinkwrap_and_fp32(nd, oracle):
     assert (np.abs(eg - eo) / eo).max() <= 1e-4
 
 
+@pytest.mark.parametrize("shape", [(100, 126), (74, 130)], ids=lambda s: "%dx%d" % s)
+def test_fused_any_size_fp32_and_n_best(nd, oracle, shape):
+    """FP32 bulk on the any-length fused passes against the FP64 oracle, and
+    the best estimates kept on the device."""
+    d = planar_state(shape, seed=15)
+    est = nd.Complex2D(*shape, double=False, data=d["x0"].astype(np.complex64))
+    g = nd.PlanarCDI(est, n_best=3)
+    xo = d["x0"].copy()
+    o = oracle.OracleCDI(xo, oracle.PLANAR, n_best=3)
+    for obj in (g, o):
+        obj.set_support(d["support"])
+        obj.set_intensity(d["intensity"])
+        obj.set_algorithm(nd.HIO)
+    eg = g.iterate(6)
+    eo = o.iterate_n(6)
+    assert rel(est.get().astype(np.complex128), xo) <= 1e-4
+    assert (np.abs(eg - eo) / eo).max() <= 1e-4
+    for k in range(3):
+        bg, e_g = g.get_best_result(k)
+        bo, e_o = o.get_best_result(k)
+        assert abs(e_g - e_o) <= 1e-4 * e_o
+        assert rel(bg.astype(np.complex128), bo) <= 1e-4
+
+
 def test_shrinkwrap_tall_field(nd, oracle):
     """nx > ny: every pixel thresholded once (the reference's loop over
     j < nx overruns the rows there)."""
@@ -186,7 +214,8 @@ def test_shrinkwrap_tall_field(nd, oracle):
     assert set(np.unique(sg)) == {0.0, 1.0}
 
 
-def test_fresnel_any_size_matches_oracle(nd, oracle):
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+def test_fresnel_any_size_matches_oracle(nd, oracle, path):
     nx, ny = 100, 126
     rng = np.random.default_rng(21)
     wf = np.exp(1j * rng.uniform(0, 0.3, (nx, ny))) * (1 + 0.1 * rng.uniform(size=(nx, ny)))
@@ -196,6 +225,7 @@ def test_fresnel_any_size_matches_oracle(nd, oracle):
     par = dict(wavelength=1.55e-10, zfd=0.909, zfs=1.6e-3, pixel=13.5e-6)
     est = nd.Complex2D(nx, ny)
     g = nd.FresnelCDI(est, wf, par["wavelength"], par["zfd"], par["zfs"], par["pixel"], 1.0)
+    g.set_path(path)
     xo = np.zeros((nx, ny), np.complex128)
     o = oracle.OracleCDI(xo, oracle.FRESNEL)
     o.fresnel_init(wf, par["wavelength"], par["zfd"], par["zfs"], par["pixel"], 1.0)
