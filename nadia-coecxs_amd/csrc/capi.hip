@@ -238,7 +238,8 @@ void* twiddles(DeviceCtx* c, int n, int prec, bool row) {
 // largest radix (13 in FP32, 7 in FP64) gets a direct mixed-radix plan
 // (radix 8 first, then 4, 2, 3, 5, 7, 11, 13); any other length Bluestein's:
 // X_k = c_k sum_j (x_j c_j) conj(c_{k-j}), c_j = exp(-i pi j^2 / n), the
-// convolution computed with a power-of-two transform of length >= 2n - 1.
+// convolution computed with transforms of the smallest length >= 2n - 1
+// whose prime factors are <= 7.
 constexpr int kAnyMaxLen = 4096;
 
 bool any_factor(int n, int maxr, std::vector<int>& rad) {
@@ -252,27 +253,46 @@ bool any_factor(int n, int maxr, std::vector<int>& rad) {
   return m == 1 && (int)rad.size() <= kAnyMaxStages;
 }
 
-// DFT (sign -1) of a power-of-two length sequence in long double, in place
-void host_fft_pow2(std::vector<long double>& re, std::vector<long double>& im) {
-  const size_t n = re.size();
-  for (size_t i = 1, j = 0; i < n; ++i) {  // bit reversal
-    size_t bit = n >> 1;
-    for (; j & bit; bit >>= 1) j ^= bit;
-    j ^= bit;
-    if (i < j) { std::swap(re[i], re[j]); std::swap(im[i], im[j]); }
-  }
-  const long double twopi = 6.283185307179586476925286766559005768L;
-  for (size_t len = 2; len <= n; len <<= 1)
-    for (size_t k = 0; k < len / 2; ++k) {
-      const long double a = -twopi * (long double)k / (long double)len;
-      const long double wr = cosl(a), wi = sinl(a);
-      for (size_t i = 0; i < n; i += len) {
-        const size_t u = i + k, v = i + k + len / 2;
-        const long double xr = re[v] * wr - im[v] * wi, xi = re[v] * wi + im[v] * wr;
-        re[v] = re[u] - xr; im[v] = im[u] - xi;
-        re[u] += xr; im[u] += xi;
-      }
+// DFT (sign -1) in long double of a length whose prime factors are <= 7:
+// recursive decimation in time on the smallest prime factor p,
+// X[k + m n/p] = sum_j W_n^{j (k + m n/p)} Y_j[k] over the p sub-transforms.
+void host_dft(std::vector<long double>& re, std::vector<long double>& im) {
+  const int n = (int)re.size();
+  if (n == 1) return;
+  int p = 2;
+  while (n % p) ++p;
+  const int m = n / p;
+  std::vector<std::vector<long double>> sr(p, std::vector<long double>(m)), si(p, std::vector<long double>(m));
+  for (int j = 0; j < p; ++j)
+    for (int k = 0; k < m; ++k) {
+      sr[j][k] = re[k * p + j];
+      si[j][k] = im[k * p + j];
     }
+  for (int j = 0; j < p; ++j) host_dft(sr[j], si[j]);
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int q = 0; q < n; ++q) {
+    const int k = q % m;
+    long double ar = 0, ai = 0;
+    for (int j = 0; j < p; ++j) {
+      const long long t = ((long long)j * q) % n;
+      const long double a = -twopi * (long double)t / (long double)n;
+      const long double c = cosl(a), sn = sinl(a);
+      ar += sr[j][k] * c - si[j][k] * sn;
+      ai += sr[j][k] * sn + si[j][k] * c;
+    }
+    re[q] = ar;
+    im[q] = ai;
+  }
+}
+
+// smallest length >= m whose prime factors are all <= 7
+int next_smooth(int m) {
+  for (int v = m;; ++v) {
+    int t = v;
+    for (int r : {2, 3, 5, 7})
+      while (t % r == 0) t /= r;
+    if (t == 1) return v;
+  }
 }
 
 const AnyPlan* any_plan(DeviceCtx* c, int n, int prec) {
@@ -290,8 +310,7 @@ const AnyPlan* any_plan(DeviceCtx* c, int n, int prec) {
   p.blue = !any_factor(n, maxr, rad);
   p.len = n;
   if (p.blue) {
-    p.len = 1;
-    while (p.len < 2 * n - 1) p.len <<= 1;
+    p.len = next_smooth(2 * n - 1);
     any_factor(p.len, maxr, rad);
   }
   p.nst = (int)rad.size();
@@ -330,7 +349,7 @@ const AnyPlan* any_plan(DeviceCtx* c, int n, int prec) {
       kr[j] = cr[j]; ki[j] = -ci[j];
       if (j > 0) { kr[p.len - j] = cr[j]; ki[p.len - j] = -ci[j]; }
     }
-    host_fft_pow2(kr, ki);
+    host_dft(kr, ki);
     for (int j = 0; j < p.len; ++j) { kr[j] /= p.len; ki[j] /= p.len; }
   }
   const size_t esz = prec == NADIA_FP64 ? 16 : 8;

@@ -136,7 +136,7 @@ struct AnyArgs {
   int nseq;             // sequences (rows or columns)
   int rows;             // 1: es == 1 (load along a sequence), 0: ss == 1 (across sequences)
   int n;                // transform length
-  int len;              // LDS sequence length: n, or Bluestein's power-of-two m >= 2n - 1
+  int len;              // LDS sequence length: n, or Bluestein's m >= 2n - 1 (prime factors <= 7)
   int cb, stride;       // sequences per workgroup, LDS elements between sequences
   int dir;              // -1 forward, +1 backward (unnormalised)
   int nst;              // stages of the len-point plan
