@@ -521,6 +521,8 @@ def main():
     ap.add_argument("--profile", default=None, metavar="LEG",
                     choices=["batched", "fresnel", "phase_diverse", "next_rows", "capacity"],
                     help="run one leg alone (for profiler runs) and print its JSON")
+    ap.add_argument("--copy-bandwidth", action="store_true",
+                    help="device copy rate (read + write) at 96 MB .. 4 GB, then exit")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="K",
                     help="instead of the benchmark, run K device copies of a 2048^2 complex64 field "
                          "(32 MiB read + 32 MiB written each) to calibrate profiler byte counters")
@@ -532,6 +534,29 @@ def main():
 
     nd = importlib.import_module("nadia-coecxs_amd")
     synth = importlib.import_module("nadia-coecxs_amd.synth")
+    if args.copy_bandwidth:
+        # the device's own copy rate (torch's copy kernel on its stream, events
+        # on that stream): source + destination inside the 256 MB Infinity
+        # Cache, then streaming from HBM
+        import torch
+        out = {}
+        for mb in (96, 256, 1024, 4096):
+            n = mb * (1 << 20) // 4
+            a = torch.ones(n, dtype=torch.float32, device="cuda")
+            b = torch.empty_like(a)
+            for _ in range(3):
+                b.copy_(a)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(20):
+                b.copy_(a)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / 20
+            out["copy_%dMB_TBs" % mb] = round(2 * n * 4 / ms / 1e9, 2)
+            del a, b
+        print(json.dumps({"copy_bandwidth": out}))
+        return
     if args.copy_calibration:
         a = nd.Complex2D(N, N, double=False)
         b = nd.Complex2D(N, N, double=False)
@@ -551,7 +576,7 @@ def main():
 
     # ---- FP32 fused, batched (the headline value) ----------------------
     # one job alone, then two independent 2048^2 jobs sharing every launch;
-    # value = the better aggregate job-iterations per second
+    # value = the two jobs' aggregate job-iterations per second
     est, p = setup(nd, data, False, nd.PATH_FUSED)
     dt = time_iterations(nd, p, args.steps, args.warmup)
     single = args.steps / dt
