@@ -76,12 +76,15 @@ shrinkwrap_threshold = 0.1
 """
 
 
-@pytest.mark.parametrize("binary,double,algs,iters,sw,tol", [
-    ("cdi_reconstructiond", True, "HIO ER", "20 10", 15, 1e-9),
-    ("cdi_reconstruction", False, "ER", "20", 0, 1e-4),
-], ids=["fp64", "fp32"])
-def test_planar_config_run(oracle, synth, tmp_path, binary, double, algs, iters, sw, tol):
-    n, seed = 128, 3
+@pytest.mark.parametrize("binary,double,algs,iters,sw,tol,n", [
+    ("cdi_reconstructiond", True, "HIO ER", "20 10", 15, 1e-9, 128),
+    ("cdi_reconstruction", False, "ER", "20", 0, 1e-4, 128),
+    # sizes without compiled passes: 120 = 8*3*5 (direct plan), 106 = 2*53 (Bluestein)
+    ("cdi_reconstructiond", True, "HIO ER", "20 10", 15, 1e-9, 120),
+    ("cdi_reconstructiond", True, "HIO ER", "20 10", 15, 1e-9, 106),
+], ids=["fp64", "fp32", "fp64-120", "fp64-106"])
+def test_planar_config_run(oracle, synth, tmp_path, binary, double, algs, iters, sw, tol, n):
+    seed = 3
     d = synth.planar_block(n=n, block=48, support=56)
     files.write_dbin(tmp_path / "intensity.dbin", d["intensity"])
     files.write_dbin(tmp_path / "support.dbin", d["support"])
