@@ -65,15 +65,27 @@ constexpr bool size_native(int n) {
 // Points per thread: row passes (contiguous transforms).  3*2^k lengths take
 // 12 points (radix-4 and radix-3 stages): 24 points left the 384 and 768
 // passes at two waves per SIMD (114-150 registers, 55 KB of LDS per
-// workgroup) and 2-3x the time per pixel of the 2^k lengths.
-constexpr int row_points(int n) { return (n % 3 == 0) ? (n < 12 ? n : 12) : (n < 8 ? n : 8); }
+// workgroup) and 2-3x the time per pixel of the 2^k lengths.  Rows of 256,
+// 512 and 1024 take 4 points (radix-4 stages, twice the threads per row):
+// their one-job passes are latency-bound, and the row pass of one 256^2 /
+// 512^2 / 1024^2 job gets 20% / 18% / 15% faster (batches unchanged); at
+// 2048 the row pass is bandwidth-bound and 4 points cost 16%.
+constexpr int row_points(int n) {
+  if (n % 3 == 0) return n < 12 ? n : 12;
+  if (n == 256 || n == 512 || n == 1024) return 4;
+  return n < 8 ? n : 8;
+}
 
 // Points per thread for column passes: 16 for 2^k >= 2048 (radix-16
 // stages, one LDS exchange fewer than the rows' radix-8 plan, and half the
 // threads per column, so twice the columns per workgroup: 2048^2 FP32
-// column pass 28.4 -> 26.5 us); the row plan below that (at 1024 radix-16
-// columns measured 4% slower).
-constexpr int col_points(int n) { return (n % 3 == 0 || n < 2048) ? row_points(n) : 16; }
+// column pass 28.4 -> 26.5 us); 8 for 2^k below that (at 1024 radix-16
+// columns measured 4% slower, 4 points 2% slower); 3*2^k as the rows.
+constexpr int col_points(int n) {
+  if (n % 3 == 0) return row_points(n);
+  if (n >= 2048) return 16;
+  return n < 8 ? n : 8;
+}
 
 // LDS pad shift of the exchanges (one pad element per 2^shift positions):
 // column tiles of 16-point plans use 16, everything else 8.
