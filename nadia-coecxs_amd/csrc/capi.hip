@@ -1087,7 +1087,7 @@ int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njob
     return 0;
   };
   if (g.kind != NADIA_FRESNEL_WF) {
-    ra.fin_n = g.ny / col_tile<T>(g.nx, cv);
+    ra.fin_n = g.ny / col_group<T>(g.nx);
     if ((rc = col(g.kind == NADIA_FRESNEL ? COL_FRESNEL : COL_PLANAR))) return rc;
     if (tcs) {  // ER with a constraint: the update pass also sums its regions
       ra.tc = *tcs;
@@ -3742,7 +3742,12 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       nadia_recon* r0 = b->jobs[kv.second[0]];
       // both twiddle tables exist before a graph capture can start
       if ((rc = ensure_twiddles(c, kv.first.nx, kv.first.ny, kv.first.prec))) return rc;
-      BatchLaunch L{kv.first, b->djobs + host.size(), r0->col_v, r0->row_v, {}, {}};
+      // the autotuner times one job alone; a launch shared by several jobs
+      // has parallelism enough for the full-width column tiles (variant 0:
+      // 64-byte row segments) unless the caller fixed the variants.  Every
+      // variant gives the same estimates and errors.
+      const int group_cv = (kv.second.size() > 1 && !r0->tile_fixed) ? 0 : r0->col_v;
+      BatchLaunch L{kv.first, b->djobs + host.size(), group_cv, r0->row_v, {}, {}};
       L.rebuild_w = all_stale;
       for (int j : kv.second) {
         host.push_back(make_jobdesc(b->jobs[j], b->jobs[j]->est->d, b->jobs[j]->W, b->jobs[j]->errs));

@@ -152,6 +152,9 @@ template <typename T> void launch_any(const AnyArgs& a, int lds_class, int threa
 template <typename T> void launch_row(const RowArgs& a, int njobs, int v, hipStream_t st);
 template <typename T> void launch_col(const ColArgs& a, int njobs, int v, hipStream_t st);
 template <typename T> int col_tile(int nx, int v);
+// columns per error-partial group of the column pass (ny / col_group
+// partials per field, the same for every tile variant)
+template <typename T> int col_group(int nx);
 template <typename T> int row_blocks(int nx, int ny, int v);
 
 

@@ -124,8 +124,11 @@ struct TileCfg {
 constexpr int kColVariants = 3;
 constexpr int kRowVariants = 2;
 
-// lengths that get more than the default variant
-constexpr bool tuned_size(int n) { return n == 512 || n == 1024 || n == 2048 || n == 4096; }
+// lengths that get more than the default variant: 256..4096, 2^k and 3*2^k
+constexpr bool tuned_size(int n) {
+  return n == 256 || n == 384 || n == 512 || n == 768 || n == 1024 || n == 1536 || n == 2048 || n == 3072 ||
+         n == 4096;
+}
 
 // Column variants: 0 default; 1 registers limited so that two workgroups
 // share a CU; 2 half the columns per workgroup.
@@ -140,6 +143,11 @@ constexpr TileCfg col_cfg(int n, int esz, int v) {
   }
   return c;
 }
+
+// Width of the column groups the error partials are summed over: the
+// narrowest column tile among the variants of the length, so that every
+// variant's tile holds whole groups and the partials do not depend on it.
+constexpr int col_group(int n, int esz) { return col_cfg(n, esz, 2).batch; }
 
 // Row variants: 0 default; 1 the same workgroup with registers limited so
 // that 8 waves (fp32; 4 in fp64) share a SIMD.  esz: bytes per complex

@@ -283,9 +283,9 @@ def test_fp32_against_fp64_oracle(nd, oracle, synth):
 @pytest.mark.parametrize("double", [False, True], ids=["fp32", "fp64"])
 def test_tile_variants_bit_identical(nd, synth, double, alg):
     """Every tile variant of the fused passes uses the same radix plan and
-    points per thread, so the estimates are bit-identical whichever variant
-    the autotuner picks (the error sums follow the tile geometry's reduction
-    tree and agree to rounding)."""
+    points per thread, and the column pass sums its error partials per fixed
+    column group, so estimates and errors are bit-identical whichever
+    variant the autotuner picks (and whichever a shared launch uses)."""
     d = synth.planar_block(n=512, block=160, support=200)
     cdt = np.complex128 if double else np.complex64
     outs = []
@@ -303,7 +303,7 @@ def test_tile_variants_bit_identical(nd, synth, double, alg):
             outs.append((est.get(), e))
     for x, e in outs[1:]:
         assert np.array_equal(x, outs[0][0])
-        assert np.abs(e - outs[0][1]).max() <= 1e-13 * np.abs(outs[0][1]).max()
+        assert np.array_equal(e, outs[0][1])
 
 
 @pytest.mark.parametrize("cv", [0, 2])
