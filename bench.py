@@ -39,12 +39,14 @@ N = 2048
 
 # Algorithmic bytes per pixel (FP32: complex 8 B, real 4 B; FP64 doubles it).
 #   column pass: read W, read sqrt(I), write W          = 2 b_c + b_r
-#   row pass (HIO): read W, x, support; write x, W      = 4 b_c + b_r
+#   row pass (HIO): read W, x, support; write x, W      = 4 b_c + 1
+#     (a support of zeros and ones, as every workload here has, is read as
+#     one byte per pixel)
 # Whole HIO iteration per pixel as counted for the roofline of the fused
 # design (row/column pair with on-chip 1-D transforms): 7 b_c + 2 b_r.
 def pass_bytes(double):
     bc, br = (16, 8) if double else (8, 4)
-    return {"col": 2 * bc + br, "row": 4 * bc + br, "iter": 7 * bc + 2 * br}
+    return {"col": 2 * bc + br, "row": 4 * bc + 1, "iter": 7 * bc + 2 * br}
 
 
 HBM_PEAK_GBS = 8000.0

@@ -525,6 +525,8 @@ struct nadia_recon {
   long long npix;
   void* arena;  // one allocation holding S, Q, W, conv, rtmp and the small per-job buffers
   void *S, *Q, *beam;
+  void* S8;     // the support as bytes (k_normalise; see JobDesc::s8)
+  bool s_soft;  // the support was softened (S8 not kept)
   int has_beam, has_intensity;
   void* W;  // 2 fields
   void* t[4];
@@ -1000,6 +1002,7 @@ JobDesc make_jobdesc(const nadia_recon* r, void* X, void* W, double* errs) {
   j.x = X;
   j.w = W;
   j.s = r->S;
+  j.s8 = r->s_soft ? nullptr : (const unsigned char*)r->S8;
   j.q = r->Q;
   j.ill = r->L;
   j.partials = r->partials;
@@ -1476,8 +1479,9 @@ void conv_window(const nadia_recon* r, double sigma, int cut, double* g) {
 int support_from_rtmp(nadia_recon* r, int soften) {
   DISPATCH(r->prec, k_real_max<T>(r->rtmp, r->npix, r->partials, kEwBlocks, S_(r)));
   k_final_max(r->partials, kEwBlocks, r->scal + 1, S_(r));
-  DISPATCH(r->prec, k_normalise<T>(r->rtmp, r->S, r->scal + 1, r->npix, S_(r)));
+  DISPATCH(r->prec, k_normalise<T>(r->rtmp, r->S, r->scal + 1, r->npix, S_(r), (unsigned char*)r->S8));
   LAUNCHCHK();
+  r->s_soft = soften != 0;
   if (soften) {
     // convolve(support, 3, 5): 11x11 window, sigma 3, left unnormalised
     ConvArgs ca{};
@@ -1726,10 +1730,10 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   // reconstruction), each on a 256-byte boundary
   {
     void** dst[] = {&r->S, &r->Q, &r->W, &r->conv, &r->rtmp, (void**)&r->partials, (void**)&r->errs,
-                    (void**)&r->scal, (void**)&r->djob, (void**)&r->tune_errs};
+                    (void**)&r->scal, (void**)&r->djob, (void**)&r->tune_errs, &r->S8};
     const size_t len[] = {rb, rb, 2 * cb, rb, rb, sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16),
                           sizeof(double) * (r->ring_cap + 8), sizeof(double) * 16, sizeof(JobDesc) * 2,
-                          sizeof(double) * 64};
+                          sizeof(double) * 64, (size_t)r->npix};
     size_t total = 0;
     for (size_t b : len) total += (b + 255) & ~(size_t)255;
     if ((rc = alloc(&r->arena, total))) {
@@ -1737,12 +1741,14 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
       return rc;
     }
     char* p = (char*)r->arena;
-    for (int i = 0; i < 10; i++) {
+    for (int i = 0; i < 11; i++) {
       *dst[i] = p;
       p += (len[i] + 255) & ~(size_t)255;
     }
   }
   (void)hipMemsetAsync(r->S, 0, rb, S_(r));
+  (void)hipMemsetAsync(r->S8, 0, r->npix, S_(r));
+  r->s_soft = false;
   (void)hipMemsetAsync(r->Q, 0, rb, S_(r));
   (void)hipMemsetAsync(r->scal, 0, sizeof(double) * 16, S_(r));
   for (int i = 0; i < n_best; i++) {
@@ -1787,9 +1793,10 @@ namespace {
 // snapshots when present.
 size_t base_bytes(long long npix, size_t esz) {
   const size_t cb = npix * esz, rb = npix * (esz / 2);
-  // (+ the 256-byte alignment of the ten buffers of the shared allocation)
-  return 4 * rb + 2 * cb + sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16) +
-         sizeof(double) * (4096 + 8) + sizeof(double) * 16 + sizeof(JobDesc) * 2 + sizeof(double) * 64 + 10 * 256;
+  // (+ the support bytes, and the 256-byte alignment of the eleven buffers
+  // of the shared allocation)
+  return 4 * rb + 2 * cb + (size_t)npix + sizeof(double) * 2 * (size_t)(kEwBlocks + 2 * 4096 + 16) +
+         sizeof(double) * (4096 + 8) + sizeof(double) * 16 + sizeof(JobDesc) * 2 + sizeof(double) * 64 + 11 * 256;
 }
 }  // namespace
 
@@ -2402,6 +2409,7 @@ int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size) {
   // src/FresnelCDI_WF.c++:177-196: integer centre (nx-1)/2
   const double centre = (double)((r->nx - 1) / 2);
   const double dz = (r->z12 / r->z23) * r->pixel;
+  r->s_soft = true;  // convolved below: the byte copy is not kept
   DISPATCH(r->prec, k_wf_disk<T>(r->S, r->nx, r->ny, centre, dz, size * z_diameter / 2.0, S_(r)));
   LAUNCHCHK();
   // convolve(support, 3, 5), left unnormalised

@@ -30,6 +30,9 @@ struct JobDesc {
   void* x;            // estimate
   void* w;            // work field(s): nfields consecutive nx*ny fields
   const void* s;      // support
+  // the support as bytes for the update pass: 0 / 1 where it is 0 / 1, 2
+  // where it is neither (read s there); null for softened supports
+  const unsigned char* s8;
   const void* q;      // sqrt(I) with the beam-stop sign
   const void* ill;    // Fresnel white field (detector plane)
   double* partials;   // this job's partial error sums
@@ -215,7 +218,10 @@ template <typename T> void k_conv(const ConvArgs& a, int cut, int* nblocks, hipS
 template <typename T> void k_real_max(const void* src, long long n, double* partials, int nblocks, hipStream_t st);
 template <typename T> void k_threshold(const void* conv, void* out, int nx, int ny, const double* maxv, double thr,
                                        hipStream_t st);
-template <typename T> void k_normalise(const void* src, void* dst, const double* maxv, long long n, hipStream_t st);
+// dst = src / max; s8 (if set) = 0 / 1 where dst is 0 / 1, 2 elsewhere
+template <typename T>
+void k_normalise(const void* src, void* dst, const double* maxv, long long n, hipStream_t st,
+                 unsigned char* s8 = nullptr);
 template <typename T> void k_wf_disk(void* s, int nx, int ny, double centre, double dz, double radius, hipStream_t st);
 // TransmissionConstraint::apply_constraint: per-region sums (partials of
 // kEwBlocks workgroups), their reduction to c_mean, and the per-pixel update

@@ -198,6 +198,42 @@ def test_soft_support_set_support_soften(nd, oracle, synth):
     assert rel(g.get_support(), o.get_support()) <= 1e-14
 
 
+@pytest.mark.parametrize("double", [True, False], ids=["fp64", "fp32"])
+def test_support_kinds_in_fused_iterations(nd, oracle, synth, double):
+    """The update pass reads a support of zeros and ones as bytes and any
+    other support in full precision.  One reconstruction goes through a
+    binary support, a fractional one given as is (no softening), a softened
+    one, a shrinkwrap (binary again) and a binary one; at every stage one HIO
+    iteration from the oracle's current estimate follows the oracle (FP64 <=
+    1e-12; FP32 <= 1e-5 of the FP64 oracle, with the GPU's shrinkwrap
+    support given to the oracle), alone and inside a batch."""
+    d = small_state(synth, n=128)
+    tol = 1e-12 if double else 1e-5
+    cdt = np.complex128 if double else np.complex64
+    stages = [("binary", d["support"], False), ("fractional", d["soft_support"], False),
+              ("softened", d["support"] * 2.0, True), ("shrinkwrap", None, False), ("binary", d["support"], False)]
+    for batched in (False, True):
+        est, g, xo, o = make_pair(nd, oracle, d, double=double, beam=True)
+        b = nd.Batch([g]) if batched else None
+        for obj in (g, o):
+            obj.set_algorithm(nd.HIO)
+        for name, sup, soften in stages:
+            est.set(xo.astype(cdt))
+            if sup is None:
+                g.apply_shrinkwrap(2.0, 0.1)
+                o.apply_shrinkwrap(2.0, 0.1)
+                if not double:
+                    o.set_support(g.get_support())
+            else:
+                g.set_support(sup, soften=soften)
+                o.set_support(sup, soften)
+            e = b.iterate(1)[0, 0] if batched else g.iterate(1)[0]
+            o.iterate()
+            x = est.get().astype(np.complex128)
+            assert rel(x, xo) <= tol, (name, batched)
+            assert abs(e - o.get_error()) <= tol * o.get_error(), (name, batched)
+
+
 def test_initialise_estimate_planar(nd, oracle, synth):
     d = small_state(synth, n=64)
     est, g, xo, o = make_pair(nd, oracle, d, soft=True)
