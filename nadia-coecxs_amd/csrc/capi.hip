@@ -994,6 +994,24 @@ bool two_fields(const nadia_recon* r) {
   return r->kind != NADIA_FRESNEL_WF && r->algorithm != NADIA_ER && r->alg[A_PFS] != 0;
 }
 
+// Split work fields (PlanarCDI and FresnelCDI with one work field and no
+// transmission constraint): the column pass leaves the projection in the
+// second field, the first keeps the row transform of x, and the update pass
+// stores x only when a sequence of iterations ends.  ER never reads x (17
+// instead of 25 bytes per pixel in the update pass); the other algorithms
+// rebuild x by one more inverse row transform instead of reading it (25
+// instead of 33 bytes), which pays for rows of up to 1024 points (64-job
+// batch of 256..1024: +5% FP32, +7% FP64) but not for 2048-point rows, whose
+// update pass is already half transform work (2048^2 HIO: -7..-10%).  A
+// fixed property of the reconstruction's set-up, so its trajectory does not
+// depend on how its iterations are split into calls or batches.
+bool split_fields(const nadia_recon* r) {
+  return (r->kind == NADIA_PLANAR || r->kind == NADIA_FRESNEL) && !two_fields(r) && !r->tc &&
+         (r->algorithm == NADIA_ER || r->ny <= 1024);
+}
+
+bool wants_n_best(const nadia_recon* r);
+
 // Host copy of the reconstruction's job descriptor for the fused passes, with
 // the estimate X and work fields W given explicitly (the autotuner points
 // them at scratch copies).
@@ -1009,6 +1027,7 @@ JobDesc make_jobdesc(const nadia_recon* r, void* X, void* W, double* errs) {
   j.errs = errs;
   for (int i = 0; i < 5; i++) j.coef[i] = r->alg[i];
   j.er = r->algorithm == NADIA_ER;
+  j.store_every = wants_n_best(r) ? 1 : 0;
   // get_error() reports the last projection the reference evaluates: P_F x
   // when a PF or PSF term exists, else P_F P_S x
   j.err_field = (r->algorithm == NADIA_ER || r->alg[A_PF] != 0 || r->alg[A_PSF] != 0) ? 0 : 1;
@@ -1026,17 +1045,20 @@ JobDesc make_jobdesc(const nadia_recon* r, void* X, void* W, double* errs) {
 
 // Reconstructions whose fused passes can share launches.
 struct GroupKey {
-  int prec, kind, nx, ny, two;
+  int prec, kind, nx, ny, two, split;
   bool operator<(const GroupKey& o) const {
     if (prec != o.prec) return prec < o.prec;
     if (kind != o.kind) return kind < o.kind;
     if (nx != o.nx) return nx < o.nx;
     if (ny != o.ny) return ny < o.ny;
-    return two < o.two;
+    if (two != o.two) return two < o.two;
+    return split < o.split;
   }
 };
 
-GroupKey group_key(const nadia_recon* r) { return GroupKey{r->prec, r->kind, r->nx, r->ny, two_fields(r) ? 1 : 0}; }
+GroupKey group_key(const nadia_recon* r) {
+  return GroupKey{r->prec, r->kind, r->nx, r->ny, two_fields(r) ? 1 : 0, split_fields(r) ? 1 : 0};
+}
 
 // Queue one fused iteration of `njobs` reconstructions sharing key g, their
 // descriptors in djobs (device); error ring slot `iter`; column / row tile
@@ -1058,6 +1080,7 @@ int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njob
   ColArgs ca{};
   ca.jobs = djobs;
   ca.nfields = g.two ? 2 : 1;
+  ca.split = g.split;
   ca.nx = g.nx;
   ca.ny = g.ny;
   ca.tw = twx;
@@ -1071,6 +1094,9 @@ int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njob
   ra.ny = g.ny;
   ra.tw = twy;
   ra.two = g.two;
+  ra.split = g.split;
+  ra.xs_first = fin_first;
+  ra.xs_count = fin_count;
   ra.post = ca.sf;
   int rc;
   auto col = [&](int mode) -> int {
@@ -3603,9 +3629,9 @@ int ensure_side(DeviceCtx* c, int n) {
 int capture_batch_graph(nadia_batch* b, DeviceCtx* c, const std::vector<BatchLaunch>& launches, int steps) {
   std::vector<long long> sig{steps, (long long)(intptr_t)b->djobs, (long long)b->gen};
   for (const BatchLaunch& L : launches) {
-    long long v[] = {L.g.prec, L.g.kind, L.g.nx, L.g.ny, L.g.two, (long long)(L.d - b->djobs),
+    long long v[] = {L.g.prec, L.g.kind, L.g.nx, L.g.ny, L.g.two, L.g.split, (long long)(L.d - b->djobs),
                      (long long)L.rem.size(), L.cv, L.rv};
-    sig.insert(sig.end(), v, v + 9);
+    sig.insert(sig.end(), v, v + 10);
     for (const nadia_recon* r : L.jobs) {
       sig.push_back((long long)(intptr_t)r);
       sig.push_back(wants_n_best(r) ? r->n_best : 0);

@@ -40,6 +40,7 @@ struct JobDesc {
   double coef[5];     // PSF, PFS, PS, PF, PI
   int er;             // error-reduction special case
   int err_field;      // field whose projection error get_error() reports
+  int store_every;    // the update pass stores x after every iteration (kept best estimates read it)
   double phase_factor;                                   // Fresnel
   const void* fx;     // Fresnel: (cos, sin) of phase_factor (xmid - x)^2 per row x, complex T
   const void* fy;     // the same per column y
@@ -76,6 +77,12 @@ struct RowArgs {
   double scale;
   // ROW_UPDATE
   int two;       // the jobs carry a second field (P_F P_S x) after the first
+  // split work fields: the column pass left the projection in the second
+  // field and the first still holds the row transform of x, from which the
+  // update rebuilds x (x itself is not read); x is stored only by the jobs
+  // [xs_first, xs_first + xs_count) (xs_count < 0: all) and those with
+  // store_every, the first field always
+  int split, xs_first, xs_count;
   double post;   // factor applied after the inverse row transform
   int fin_n;     // number of column-pass blocks per field (error finalisation)
   // ROW_TC, ROW_UPDTC
@@ -99,6 +106,9 @@ struct ColArgs {
   // COL_WF1: the previous iteration's error (ring slot iter - 1, from fin_n
   // row-pass partial pairs) is reduced by each job's first workgroup
   int iter, fin_n;
+  // COL_PLANAR / COL_FRESNEL: the projection goes to the job's second work
+  // field (split work fields, see RowArgs), not back into the first
+  int split;
 };
 
 struct ConvArgs {
