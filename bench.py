@@ -137,7 +137,12 @@ def leg_accuracy_planar(nd, synth, data, hio=1000, every=20, er=100):
             rng = np.random.default_rng(99)
             d["x0"] = data["x0"] * (1 + eps * rng.standard_normal(data["x0"].shape))
         est, p = setup(nd, d, double, nd.PATH_FUSED)
-        p.iterate_async(1)  # tile-variant choice and first transform outside the clock
+        # tile-variant choices (HIO, and ER, whose work-field layout has its
+        # own) and the first transforms outside the clock
+        p.iterate_async(1)
+        p.set_algorithm(nd.ER)
+        p.iterate_async(1)
+        p.set_algorithm(nd.HIO)
         nd.synchronize()
         est.set(d["x0"].astype(np.complex128 if double else np.complex64))
         t0 = time.perf_counter()
@@ -482,6 +487,8 @@ def profile_leg(nd, synth, args):
         out = leg_phase_diverse(nd, synth)
     elif leg == "next_rows":
         out = leg_next_rows(nd, synth, synth.planar_star(n=N))
+    elif leg == "accuracy":
+        out = leg_accuracy_planar(nd, synth, synth.planar_star(n=N))
     elif leg == "capacity":
         serve = importlib.import_module("nadia-coecxs_amd.serve")
         reqs = serve.poisson_stream(count=args.serve_requests, rate=args.serve_rate)
@@ -521,7 +528,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--profile", default=None, metavar="LEG",
-                    choices=["batched", "fresnel", "phase_diverse", "next_rows", "capacity"],
+                    choices=["batched", "fresnel", "phase_diverse", "next_rows", "capacity", "accuracy"],
                     help="run one leg alone (for profiler runs) and print its JSON")
     ap.add_argument("--copy-bandwidth", action="store_true",
                     help="device copy rate (read + write) at 96 MB .. 4 GB, then exit")

@@ -1493,11 +1493,12 @@ int update_n_best(nadia_recon* r, const double* err, hipStream_t st) {
 // BaseCDI::convolve's window g(i, j) = exp(-(i^2 + j^2) / (2 sigma^2)),
 // 0 <= i, j <= cut (src/BaseCDI.c++:441-447), rounded to the field's real
 // type as the reference stores it
-void conv_window(const nadia_recon* r, double sigma, int cut, double* g) {
+void conv_window(const nadia_recon* r, double sigma, int cut, double* g, float* gf) {
   for (int i = 0; i <= cut; i++)
     for (int j = 0; j <= cut; j++) {
       const double v = exp(-1 * (i * i + j * j) / (2.0 * sigma * sigma));
       g[i * (cut + 1) + j] = r->prec == NADIA_FP64 ? v : (double)(float)v;
+      gf[i * (cut + 1) + j] = (float)v;
     }
 }
 
@@ -1511,7 +1512,7 @@ int support_from_rtmp(nadia_recon* r, int soften) {
   if (soften) {
     // convolve(support, 3, 5): 11x11 window, sigma 3, left unnormalised
     ConvArgs ca{};
-    conv_window(r, 3.0, 5, ca.g);
+    conv_window(r, 3.0, 5, ca.g, ca.gf);
     ca.src = r->S;
     ca.src_complex = 0;
     ca.dst = r->conv;
@@ -2036,7 +2037,7 @@ int nadia_recon_apply_shrinkwrap(nadia_recon* r, double gauss_width, double thre
   // queued on the device (the window travels in the kernel arguments): the
   // call returns without waiting.
   ConvArgs ca{};
-  conv_window(r, gauss_width, 4, ca.g);
+  conv_window(r, gauss_width, 4, ca.g, ca.gf);
   ca.src = r->est->d;
   ca.src_complex = 1;
   ca.dst = r->conv;
@@ -2441,7 +2442,7 @@ int nadia_wf_set_support(nadia_recon* r, double z_diameter, double size) {
   // convolve(support, 3, 5), left unnormalised
   HIPCHK(hipMemcpyAsync(r->rtmp, r->S, r->npix * r->rsz, hipMemcpyDeviceToDevice, S_(r)));
   ConvArgs ca{};
-  conv_window(r, 3.0, 5, ca.g);
+  conv_window(r, 3.0, 5, ca.g, ca.gf);
   ca.src = r->rtmp;
   ca.src_complex = 0;
   ca.dst = r->S;

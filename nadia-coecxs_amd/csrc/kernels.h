@@ -117,8 +117,10 @@ struct ConvArgs {
   void* dst;        // real
   int nx, ny;
   // (cut+1)^2 window values, rounded to the field's real type, passed by
-  // value (no upload, nothing for the host to keep alive)
+  // value (no upload, nothing for the host to keep alive); gf: the same
+  // values as floats, read by the single-precision kernels
   double g[36];
+  float gf[36];
   double* partials; // one max per block
 };
 

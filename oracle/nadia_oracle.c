@@ -326,7 +326,9 @@ void ocdi_destroy(ocdi *o) {
 }
 
 /* BaseCDI::convolve: NADIA/src/BaseCDI.c++:432-474 (unnormalised truncated
- * Gaussian, zero outside the array, double accumulator, window stored in real) */
+ * Gaussian, zero outside the array, window stored in real; each product
+ * array.get() * gauss_dist.get() is formed in real, float * float in the
+ * single-precision build, and added to a double accumulator) */
 void ocdi_convolve(ocdi *o, real *array, double gauss_width, int cut) {
   int nx = o->nx, ny = o->ny;
   real *tmp = calloc_r(NPIX(o));
@@ -342,7 +344,7 @@ void ocdi_convolve(ocdi *o, real *array, double gauss_width, int cut) {
       for (int i2 = i - cut; i2 <= i + cut; i2++)
         for (int j2 = j - cut; j2 <= j + cut; j2++)
           if (i2 < nx && i2 >= 0 && j2 >= 0 && j2 < ny)
-            v += (double)array[(size_t)i2 * ny + j2] * (double)g[abs(i - i2) * (cut + 1) + abs(j - j2)];
+            v += (double)(real)(array[(size_t)i2 * ny + j2] * g[abs(i - i2) * (cut + 1) + abs(j - j2)]);
       tmp[(size_t)i * ny + j] = (real)v;
     }
   memcpy(array, tmp, sizeof(real) * NPIX(o));

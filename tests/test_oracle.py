@@ -266,6 +266,37 @@ def test_shrinkwrap_convolution_scipy(oracle):
         assert rel(o.convolve(a, sigma, cut), ref) < 1e-14
 
 
+def test_shrinkwrap_convolution_fp32_products(oracle):
+    """Single-precision build: BaseCDI::convolve forms each product
+    array.get() * gauss_dist.get() of two floats in float and adds it to a
+    double (NADIA/src/BaseCDI.c++:455-458).  Restated as shift-and-add over
+    the taps, di then dj ascending (the per-pixel order of the reference's
+    loops), each tap's products rounded to float32; the same sum with the
+    exact (double) products must differ, so the check tells the two apart."""
+    rng = np.random.default_rng(21)
+    n, m = 40, 36
+    x = (rng.random((n, m)) + 1j * rng.random((n, m))).astype(np.complex64)
+    of = oracle.OracleCDI(x.copy(), double=False)
+    a = (rng.random((n, m)) * 3.0).astype(np.float32)
+    for sigma, cut in ((1.5, 4), (3.0, 5)):
+        i = np.arange(cut + 1)
+        g = np.exp(-(i[:, None] ** 2 + i[None, :] ** 2) / (2.0 * sigma * sigma)).astype(np.float32)
+        ap = np.zeros((n + 2 * cut, m + 2 * cut), np.float32)
+        ap[cut:cut + n, cut:cut + m] = a
+        acc_f = np.zeros((n, m))
+        acc_d = np.zeros((n, m))
+        for di in range(-cut, cut + 1):
+            for dj in range(-cut, cut + 1):
+                tap = ap[cut + di:cut + di + n, cut + dj:cut + dj + m]
+                w = g[abs(di), abs(dj)]
+                acc_f += (tap * w).astype(np.float64)  # float32 product
+                acc_d += tap.astype(np.float64) * np.float64(w)
+        got = of.convolve(a, sigma, cut)
+        assert got.dtype == np.float32
+        assert np.array_equal(got, acc_f.astype(np.float32))
+        assert not np.array_equal(got, acc_d.astype(np.float32))
+
+
 def test_shrinkwrap_threshold(oracle):
     from scipy.signal import convolve2d
 
