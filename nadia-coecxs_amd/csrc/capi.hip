@@ -2047,6 +2047,19 @@ int nadia_recon_apply_shrinkwrap(nadia_recon* r, double gauss_width, double thre
   int nb = 0;
   DISPATCH(r->prec, k_conv<T>(ca, 4, &nb, S_(r)));
   k_final_max(r->partials, nb, r->scal, S_(r));
+  if (threshold <= 1.0 && r->ny <= r->nx) {
+    // Every pixel is thresholded (apply_threshold's j < nx covers each row)
+    // and the largest value stays 1 (it is not below threshold * max when
+    // threshold <= 1; with a NaN max every comparison fails and all are 1),
+    // so set_support's maximum is 1 and its division the identity: the
+    // threshold writes the support and its byte copy directly.
+    DISPATCH(r->prec, k_threshold<T>(r->conv, r->S, r->nx, r->ny, r->scal, threshold, S_(r),
+                                     (unsigned char*)r->S8));
+    LAUNCHCHK();
+    r->s_soft = false;
+    r->s_version++;
+    return 0;
+  }
   DISPATCH(r->prec, k_threshold<T>(r->conv, r->rtmp, r->nx, r->ny, r->scal, threshold, S_(r)));
   LAUNCHCHK();
   return support_from_rtmp(r, 0);

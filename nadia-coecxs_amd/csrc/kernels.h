@@ -229,7 +229,7 @@ template <typename T> void k_trans_from_esw(const void* esw, const void* ill, vo
 template <typename T> void k_conv(const ConvArgs& a, int cut, int* nblocks, hipStream_t st);
 template <typename T> void k_real_max(const void* src, long long n, double* partials, int nblocks, hipStream_t st);
 template <typename T> void k_threshold(const void* conv, void* out, int nx, int ny, const double* maxv, double thr,
-                                       hipStream_t st);
+                                       hipStream_t st, unsigned char* s8 = nullptr);
 // dst = src / max; s8 (if set) = 0 / 1 where dst is 0 / 1, 2 elsewhere
 template <typename T>
 void k_normalise(const void* src, void* dst, const double* maxv, long long n, hipStream_t st,
