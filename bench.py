@@ -487,6 +487,8 @@ def profile_leg(nd, synth, args):
         out = leg_phase_diverse(nd, synth)
     elif leg == "next_rows":
         out = leg_next_rows(nd, synth, synth.planar_star(n=N))
+    elif leg == "reference_example":
+        out = leg_reference_example(nd, synth)
     elif leg == "accuracy":
         out = leg_accuracy_planar(nd, synth, synth.planar_star(n=N))
     elif leg == "capacity":
@@ -528,7 +530,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=40.0)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--profile", default=None, metavar="LEG",
-                    choices=["batched", "fresnel", "phase_diverse", "next_rows", "capacity", "accuracy"],
+                    choices=["batched", "fresnel", "phase_diverse", "next_rows", "capacity", "accuracy",
+                             "reference_example"],
                     help="run one leg alone (for profiler runs) and print its JSON")
     ap.add_argument("--copy-bandwidth", action="store_true",
                     help="device copy rate (read + write) at 96 MB .. 4 GB, then exit")

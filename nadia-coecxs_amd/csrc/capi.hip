@@ -1028,6 +1028,12 @@ JobDesc make_jobdesc(const nadia_recon* r, void* X, void* W, double* errs) {
   for (int i = 0; i < 5; i++) j.coef[i] = r->alg[i];
   j.er = r->algorithm == NADIA_ER;
   j.store_every = wants_n_best(r) ? 1 : 0;
+  if (wants_n_best(r)) {
+    j.nb_err = r->dbest_err;
+    j.nb_perm = r->dbest_perm;
+    j.nb_target = r->dbest_target;
+    j.nb_n = r->n_best;
+  }
   // get_error() reports the last projection the reference evaluates: P_F x
   // when a PF or PSF term exists, else P_F P_S x
   j.err_field = (r->algorithm == NADIA_ER || r->alg[A_PF] != 0 || r->alg[A_PSF] != 0) ? 0 : 1;
@@ -1072,7 +1078,7 @@ GroupKey group_key(const nadia_recon* r) {
 template <typename T>
 int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njobs, int iter, int cv, int rv,
                  nadia_recon* timer, std::vector<Ev>* evs, hipStream_t st = nullptr, const TcDev* tcs = nullptr,
-                 double* tc_part = nullptr, int fin_first = 0, int fin_count = -1) {
+                 double* tc_part = nullptr, int fin_first = 0, int fin_count = -1, int rank = 0) {
   void* twy = twiddles(c, g.ny, g.prec, true);
   void* twx = twiddles(c, g.nx, g.prec, false);
   if (!twx || !twy) return fail(NADIA_EHIP, "twiddle table allocation failed");
@@ -1097,6 +1103,7 @@ int launch_fused(DeviceCtx* c, const GroupKey& g, const JobDesc* djobs, int njob
   ra.split = g.split;
   ra.xs_first = fin_first;
   ra.xs_count = fin_count;
+  ra.rank = rank;
   ra.post = ca.sf;
   int rc;
   auto col = [&](int mode) -> int {
@@ -1276,6 +1283,7 @@ void fused_done(nadia_recon* r, int n) {
 
 bool wants_n_best(const nadia_recon* r);
 int update_n_best(nadia_recon* r, const double* err, hipStream_t st);
+int n_best_copy(nadia_recon* r, hipStream_t st);
 
 template <typename T>
 int iterate_fused(nadia_recon* r, int n) {
@@ -1285,10 +1293,12 @@ int iterate_fused(nadia_recon* r, int n) {
   const GroupKey g = group_key(r);
   const bool nb = wants_n_best(r);
   for (int k = 0; k < n; ++k) {
+    // the update pass ranks the error for the kept best estimates; the
+    // snapshot copy follows it
     if ((rc = launch_fused<T>(r->ctx, g, r->djob, 1, k, r->col_v, r->row_v, r, r->timing ? &evs : nullptr,
-                              nullptr, nullptr, nullptr, 0, k == n - 1 ? 1 : 0)))
+                              nullptr, nullptr, nullptr, 0, k == n - 1 ? 1 : 0, 1)))
       return rc;
-    if (nb && (rc = update_n_best(r, r->errs + k, S_(r)))) return rc;
+    if (nb && (rc = n_best_copy(r, S_(r)))) return rc;
   }
   fused_done(r, n);
   return collect_events(r, evs);
@@ -1486,6 +1496,16 @@ int update_n_best(nadia_recon* r, const double* err, hipStream_t st) {
   if (!wants_n_best(r)) return 0;
   k_nbest(err, r->dbest_err, r->dbest_perm, r->n_best, r->dbest_target, r->est->d, r->dbest_bufs,
           (long long)(r->npix * r->esz), st);
+  LAUNCHCHK();
+  return 0;
+}
+
+// The snapshot copy of update_n_best after a fused update pass launched with
+// rank = 1, which ranked the error itself (the job descriptor carries the
+// kept errors, the rank -> buffer map and the copy target).
+int n_best_copy(nadia_recon* r, hipStream_t st) {
+  if (!wants_n_best(r)) return 0;
+  k_nbest_copy(r->dbest_target, r->est->d, r->dbest_bufs, (long long)(r->npix * r->esz), st);
   LAUNCHCHK();
   return 0;
 }
@@ -3609,12 +3629,13 @@ struct BatchLaunch {
   std::vector<int> rem;
   std::vector<nadia_recon*> jobs;  // in descriptor order
   bool rebuild_w = false;          // every job's W is rebuilt in one launch first
-  // after iteration k (ring slot `slot`) of the first a jobs: their n-best
-  // bookkeeping, on the group's stream
-  int n_best_after(int a, int slot, hipStream_t st) const {
+  // after iteration k of the first a jobs: the snapshot copies of their
+  // kept best estimates (the shared update pass, launched with rank = 1,
+  // ranked each job's error), on the group's stream
+  int n_best_after(int a, hipStream_t st) const {
     int rc;
     for (int q = 0; q < a; ++q)
-      if (wants_n_best(jobs[q]) && (rc = update_n_best(jobs[q], jobs[q]->errs + slot, st))) return rc;
+      if (wants_n_best(jobs[q]) && (rc = n_best_copy(jobs[q], st))) return rc;
     return 0;
   }
   int active(int k) const {
@@ -3673,8 +3694,8 @@ int capture_batch_graph(nadia_batch* b, DeviceCtx* c, const std::vector<BatchLau
     }
     for (int k = 0; k < steps && !rc; ++k) {
       DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, (int)L.rem.size(), k, L.cv, L.rv, nullptr, nullptr, s2,
-                                              nullptr, nullptr, 0, k == steps - 1 ? -1 : 0));
-      if (!rc) rc = L.n_best_after((int)L.rem.size(), k, s2);
+                                              nullptr, nullptr, 0, k == steps - 1 ? -1 : 0, 1));
+      if (!rc) rc = L.n_best_after((int)L.rem.size(), s2);
     }
     if (!rc && i > 0 && hipEventRecord(c->join[i - 1], s2) != hipSuccess)
       rc = fail(NADIA_EHIP, "event record in capture failed");
@@ -3869,8 +3890,8 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
         // the jobs whose last iteration this is: [active(k + 1), a)
         const int a_next = k + 1 < chunk ? L.active(k + 1) : 0;
         DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, k - eager_from, L.cv, L.rv, nullptr, nullptr, s2,
-                                                nullptr, nullptr, a_next, a - a_next));
-        if (rc || (rc = L.n_best_after(a, k - eager_from, s2))) return rc;
+                                                nullptr, nullptr, a_next, a - a_next, 1));
+        if (rc || (rc = L.n_best_after(a, s2))) return rc;
       }
     }
     // jobs outside the shared launches: ER with a transmission constraint

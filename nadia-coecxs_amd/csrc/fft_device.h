@@ -294,6 +294,28 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
 // threads of the workgroup sum them in a fixed order and the rest add
 // zeros, so the value does not depend on which kernel (and workgroup size)
 // reduces it.  Result in out[0]; `red` as for block_sum2.
+// BaseCDI::update_n_best's ranking (NADIA/src/BaseCDI.c++:390-413), one
+// thread: error e ranks against the n kept errors (ascending); on a place,
+// the buffer of the worst entry becomes the new one (perm maps rank ->
+// buffer) and *target names it for the snapshot copy, else *target = -1.  A
+// NaN error ranks first, as in the reference's `>` test.
+__device__ __forceinline__ void nbest_rank(double e, double* best_err, int* perm, int n, int* target) {
+  int place = 0;
+  while (place < n && e > best_err[place]) place++;
+  if (place < n) {
+    const int t = perm[n - 1];
+    for (int i = n - 1; i > place; i--) {
+      best_err[i] = best_err[i - 1];
+      perm[i] = perm[i - 1];
+    }
+    best_err[place] = e;
+    perm[place] = t;
+    *target = t;
+  } else {
+    *target = -1;
+  }
+}
+
 __device__ __forceinline__ void finalize_err256(const double* p, int n, double* out, double* red) {
   double d = 0, m = 0;
   if (threadIdx.x < 256)

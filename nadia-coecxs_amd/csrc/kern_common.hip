@@ -69,27 +69,12 @@ void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t s
 }
 
 // BaseCDI::update_n_best (NADIA/src/BaseCDI.c++:390-413) on the device: the
-// error of the iteration just run (err) ranks against the kept errors; on a
-// place, the buffer of the worst entry becomes the new one (perm maps rank ->
-// buffer) and *target names it for the copy, else *target = -1.  A NaN error
-// ranks first, as in the reference's `>` test.
+// error of the iteration just run (err) ranks against the kept errors
+// (nbest_rank); the fused update pass ranks in its own error workgroup
+// instead and only the copy is launched.
 __global__ void nbest_rank_kernel(const double* err, double* best_err, int* perm, int n, int* target) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double e = *err;
-  int place = 0;
-  while (place < n && e > best_err[place]) place++;
-  if (place < n) {
-    const int t = perm[n - 1];
-    for (int i = n - 1; i > place; i--) {
-      best_err[i] = best_err[i - 1];
-      perm[i] = perm[i - 1];
-    }
-    best_err[place] = e;
-    perm[place] = t;
-    *target = t;
-  } else {
-    *target = -1;
-  }
+  nbest_rank(*err, best_err, perm, n, target);
 }
 
 // the snapshot copy of the estimate into buffer *target (when >= 0)
@@ -105,6 +90,10 @@ __global__ void nbest_copy_kernel(const unsigned long long* src, unsigned long l
 void k_nbest(const double* err, double* best_err, int* perm, int n, int* target, const void* src, void* const* bufs,
              long long bytes, hipStream_t st) {
   hipLaunchKernelGGL(nbest_rank_kernel, dim3(1), dim3(64), 0, st, err, best_err, perm, n, target);
+  k_nbest_copy(target, src, bufs, bytes, st);
+}
+
+void k_nbest_copy(const int* target, const void* src, void* const* bufs, long long bytes, hipStream_t st) {
   const long long n8 = bytes / 8;
   long long blocks = (n8 + 255) / 256;
   if (blocks > 2048) blocks = 2048;

@@ -41,6 +41,12 @@ struct JobDesc {
   int er;             // error-reduction special case
   int err_field;      // field whose projection error get_error() reports
   int store_every;    // the update pass stores x after every iteration (kept best estimates read it)
+  // kept best estimates (BaseCDI::update_n_best): errors by rank, rank ->
+  // buffer map, the buffer the next snapshot goes to (-1: none), how many
+  double* nb_err;
+  int* nb_perm;
+  int* nb_target;
+  int nb_n;
   double phase_factor;                                   // Fresnel
   const void* fx;     // Fresnel: (cos, sin) of phase_factor (xmid - x)^2 per row x, complex T
   const void* fy;     // the same per column y
@@ -85,6 +91,10 @@ struct RowArgs {
   int split, xs_first, xs_count;
   double post;   // factor applied after the inverse row transform
   int fin_n;     // number of column-pass blocks per field (error finalisation)
+  // ROW_UPDATE: the workgroup that publishes a job's error also ranks it
+  // against the job's kept best errors (nb_n > 0); the snapshot copy is a
+  // launch of its own after the pass
+  int rank;
   // ROW_TC, ROW_UPDTC
   TcDev tc;
   double* tc_part;  // ROW_UPDTC: per-workgroup region sums, [2*(block*nregions + k)]
@@ -294,6 +304,8 @@ void k_copy_doubles(const CopyList& l, hipStream_t st);
 // the estimate `src` into bufs[new rank's buffer] when it placed
 void k_nbest(const double* err, double* best_err, int* perm, int n, int* target, const void* src, void* const* bufs,
              long long bytes, hipStream_t st);
+// the copy alone, after an update pass that ranked the error itself
+void k_nbest_copy(const int* target, const void* src, void* const* bufs, long long bytes, hipStream_t st);
 void k_final_sum2(const double* partials, int n, double* out_ratio, double* out_pair, hipStream_t st);
 // per-job final error reduction of the white-field row pass (grid = njobs)
 void k_final_jobs(const JobDesc* jobs, int njobs, int n, int iter, hipStream_t st);
