@@ -290,10 +290,6 @@ __device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
   }
 }
 
-// The error of an iteration from n partial (diff, norm) pairs: the first 256
-// threads of the workgroup sum them in a fixed order and the rest add
-// zeros, so the value does not depend on which kernel (and workgroup size)
-// reduces it.  Result in out[0]; `red` as for block_sum2.
 // BaseCDI::update_n_best's ranking (NADIA/src/BaseCDI.c++:390-413), one
 // thread: error e ranks against the n kept errors (ascending); on a place,
 // the buffer of the worst entry becomes the new one (perm maps rank ->
@@ -316,6 +312,10 @@ __device__ __forceinline__ void nbest_rank(double e, double* best_err, int* perm
   }
 }
 
+// The error of an iteration from n partial (diff, norm) pairs: the first 256
+// threads of the workgroup sum them in a fixed order and the rest add
+// zeros, so the value does not depend on which kernel (and workgroup size)
+// reduces it.  Result in out[0]; `red` as for block_sum2.
 __device__ __forceinline__ void finalize_err256(const double* p, int n, double* out, double* red) {
   double d = 0, m = 0;
   if (threadIdx.x < 256)
