@@ -190,6 +190,37 @@ def test_shrinkwrap_fp64(nd, oracle, synth):
     assert rel(est.get(), xo) <= 1e-10
 
 
+@pytest.mark.parametrize("double", [True, False], ids=["fp64", "fp32"])
+def test_shrinkwrap_threshold_edges(nd, oracle, synth, double):
+    """Thresholds at the edges of the threshold kernel's direct support
+    write (every pixel thresholded and threshold <= 1): exactly 1 keeps only
+    the maximum's pixels (the support's maximum is still 1), 1.5 zeroes every
+    pixel and set_support then divides 0 by 0 as the reference does (NaN
+    support), through the maximum and normalise launches.  Bit-equal to the
+    oracle of the same precision from the same estimate."""
+    n = 96
+    rng = np.random.default_rng(37)
+    cdt = np.complex128 if double else np.complex64
+    x = (rng.random((n, n)) * np.exp(2j * np.pi * rng.random((n, n)))).astype(cdt)
+    d = small_state(synth, n=n)
+    for thr in (1.0, 1.5):
+        est = nd.Complex2D(n, n, double=double, data=x)
+        g = nd.PlanarCDI(est)
+        o = oracle.OracleCDI(x.copy(), oracle.PLANAR, double=double)
+        for obj in (g, o):
+            obj.set_support(d["support"])
+            obj.set_intensity(d["intensity"])
+        g.apply_shrinkwrap(2.0, thr)
+        o.apply_shrinkwrap(2.0, thr)
+        sg, so = g.get_support(), o.get_support()
+        assert np.array_equal(sg, so, equal_nan=True), thr
+        if thr == 1.0:
+            assert 0 < sg.sum() < n * n and set(np.unique(sg)) == {0.0, 1.0}
+        else:
+            assert np.isnan(sg).all()
+        del g, est
+
+
 def test_soft_support_set_support_soften(nd, oracle, synth):
     d = small_state(synth, n=64)
     est, g, xo, o = make_pair(nd, oracle, d)
