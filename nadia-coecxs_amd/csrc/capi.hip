@@ -1025,6 +1025,9 @@ JobDesc make_jobdesc(const nadia_recon* r, void* X, void* W, double* errs) {
   j.ill = r->L;
   j.partials = r->partials;
   j.errs = errs;
+  // the fused update pass also writes each error into get_error()'s slot
+  // (not the tuner's scratch descriptors, whose errors go to tune_errs)
+  j.cur_err = errs == r->errs ? r->scal + 8 : nullptr;
   for (int i = 0; i < 5; i++) j.coef[i] = r->alg[i];
   j.er = r->algorithm == NADIA_ER;
   j.store_every = wants_n_best(r) ? 1 : 0;
@@ -1277,7 +1280,11 @@ void fused_done(nadia_recon* r, int n) {
     r->w2_version = r->est->version;
     r->w2_sversion = r->s_version;
   }
-  r->last_err = r->errs + (n - 1);
+  // PlanarCDI / FresnelCDI update passes wrote every error into get_error()'s
+  // slot as well (no copy out of the ring needed); other paths leave it in
+  // the ring
+  const bool direct = fused_ok(r) && r->kind != NADIA_FRESNEL_WF;
+  r->last_err = direct ? r->scal + 8 : r->errs + (n - 1);
   r->h_err_valid = false;
 }
 

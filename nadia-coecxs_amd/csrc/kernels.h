@@ -37,6 +37,7 @@ struct JobDesc {
   const void* ill;    // Fresnel white field (detector plane)
   double* partials;   // this job's partial error sums
   double* errs;       // this job's error ring (slot = iteration index)
+  double* cur_err;    // get_error()'s slot: the update pass writes each error there too (null: not)
   double coef[5];     // PSF, PFS, PS, PF, PI
   int er;             // error-reduction special case
   int err_field;      // field whose projection error get_error() reports
