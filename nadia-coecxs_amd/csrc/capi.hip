@@ -3636,6 +3636,7 @@ struct BatchLaunch {
   std::vector<int> rem;
   std::vector<nadia_recon*> jobs;  // in descriptor order
   bool rebuild_w = false;          // every job's W is rebuilt in one launch first
+  bool record = false;             // time the column variants on this call's first iterations
   // after iteration k of the first a jobs: the snapshot copies of their
   // kept best estimates (the shared update pass, launched with rank = 1,
   // ranked each job's error), on the group's stream
@@ -3651,6 +3652,62 @@ struct BatchLaunch {
     return a;
   }
 };
+
+// ---- column tiles of shared launches ----------------------------------------
+// The tuner above times one job alone.  A launch shared by several jobs can
+// run faster on other column tiles than the job alone (2048^2 FP32: two jobs
+// on half-width tiles, three or more on full-width ones; groups of 1024^2 on
+// the register-limited ones), so the first eager iterations of a launch group
+// of a new (key, job count) run the column variants in turn — every variant
+// gives the same estimates and errors — in blocks of kGroupBlock iterations
+// timed by events, in the order 0 1 2 2 1 0 (a clock still rising favours no
+// variant), and a later call, once those events have completed, keeps the
+// variant with the fastest block.  Nothing waits for the device: until then
+// the group runs the full-width tiles.  Only a batch that is one launch
+// group times and uses them: groups that run side by side disturb each
+// other's timings (the 64-job batch of 256..1024 lost 1% in FP32 and 5% in
+// FP64 with choices timed there), so they keep the full-width tiles.
+struct GroupTune {
+  int cv = -1;  // the chosen variant (-1: not yet)
+  std::vector<hipEvent_t> ev;  // start/stop pairs around the timed blocks
+  std::vector<int> trial_cv;   // the variant of each block
+};
+std::map<std::pair<GroupKey, int>, GroupTune> g_group_tune;
+constexpr int kGroupBlock = 3;  // iterations per timed block
+constexpr int kGroupTimed = 2 * kColVariants * kGroupBlock;
+
+// the variant of timed block q (0 1 2 2 1 0)
+int group_trial_variant(int q) { return q < kColVariants ? q : 2 * kColVariants - 1 - q; }
+
+// The column variant a shared launch of `njobs` jobs of key g uses on this
+// call, and whether this call records the variants' trial iterations.
+int group_variant(const GroupKey& g, int njobs, bool* record) {
+  *record = false;
+  if (njobs < 2 || !tuned_size(g.nx) || env_int("NADIA_TUNE", 1) == 0) return 0;
+  std::lock_guard<std::mutex> lk(g_mu);
+  GroupTune& t = g_group_tune[std::make_pair(g, njobs)];
+  if (t.cv >= 0) return t.cv;
+  if (t.ev.empty()) {
+    *record = true;
+    return 0;
+  }
+  for (hipEvent_t e : t.ev)
+    if (hipEventQuery(e) != hipSuccess) return 0;  // still running (or failed: keep the default)
+  std::vector<float> best(kColVariants, 1e30f);
+  for (size_t i = 0; i < t.trial_cv.size(); ++i) {
+    float ms = 1e30f;
+    if (hipEventElapsedTime(&ms, t.ev[2 * i], t.ev[2 * i + 1]) == hipSuccess && ms < best[t.trial_cv[i]])
+      best[t.trial_cv[i]] = ms;
+  }
+  int cv = 0;
+  for (int v = 1; v < kColVariants; ++v)
+    if (best[v] < best[cv]) cv = v;
+  for (hipEvent_t e : t.ev) (void)hipEventDestroy(e);
+  t.ev.clear();
+  t.trial_cv.clear();
+  t.cv = cv;
+  return cv;
+}
 
 int ensure_side(DeviceCtx* c, int n) {
   if (!c->fork) HIPCHK(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
@@ -3819,12 +3876,16 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       // both twiddle tables exist before a graph capture can start
       if ((rc = ensure_twiddles(c, kv.first.nx, kv.first.ny, kv.first.prec))) return rc;
       // the autotuner times one job alone; a launch shared by several jobs
-      // has parallelism enough for the full-width column tiles (variant 0:
-      // 64-byte row segments) unless the caller fixed the variants.  Every
+      // takes the column tiles timed for its key and job count (full-width
+      // tiles until then), unless the caller fixed the variants.  Every
       // variant gives the same estimates and errors.
-      const int group_cv = (kv.second.size() > 1 && !r0->tile_fixed) ? 0 : r0->col_v;
+      bool record = false;
+      const int group_cv = (kv.second.size() > 1 && !r0->tile_fixed)
+                               ? (groups.size() == 1 ? group_variant(kv.first, (int)kv.second.size(), &record) : 0)
+                               : r0->col_v;
       BatchLaunch L{kv.first, b->djobs + host.size(), group_cv, r0->row_v, {}, {}};
       L.rebuild_w = all_stale;
+      L.record = record;
       for (int j : kv.second) {
         host.push_back(make_jobdesc(b->jobs[j], b->jobs[j]->est->d, b->jobs[j]->W, b->jobs[j]->errs));
         L.rem.push_back(rem[j]);
@@ -3891,14 +3952,43 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
     for (int i = 0; i < nl; ++i) {
       const BatchLaunch& L = launches[i];
       hipStream_t s2 = (fork && i > 0) ? c->side[i - 1] : c->stream;
+      // timing the column variants: the first kGroupTimed iterations, when
+      // every job of the group runs all of them, in blocks
+      const int njl = (int)L.jobs.size();
+      const bool rec = L.record && chunk - eager_from >= kGroupTimed && L.active(eager_from + kGroupTimed - 1) == njl;
+      std::vector<hipEvent_t> tev;
+      std::vector<int> tcv;
       for (int k = eager_from; k < chunk; ++k) {
         const int a = L.active(k);
         if (a == 0) break;  // jobs are sorted by remaining count: none later either
         // the jobs whose last iteration this is: [active(k + 1), a)
         const int a_next = k + 1 < chunk ? L.active(k + 1) : 0;
-        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, k - eager_from, L.cv, L.rv, nullptr, nullptr, s2,
+        const int q = k - eager_from;
+        const bool timed = rec && q < kGroupTimed;
+        const int cv = timed ? group_trial_variant(q / kGroupBlock) : L.cv;
+        if (timed && q % kGroupBlock == 0) {
+          hipEvent_t e0, e1;
+          HIPCHK(hipEventCreate(&e0));
+          HIPCHK(hipEventCreate(&e1));
+          tev.push_back(e0);
+          tev.push_back(e1);
+          tcv.push_back(cv);
+          HIPCHK(hipEventRecord(e0, s2));
+        }
+        DISPATCH(L.g.prec, rc = launch_fused<T>(c, L.g, L.d, a, k - eager_from, cv, L.rv, nullptr, nullptr, s2,
                                                 nullptr, nullptr, a_next, a - a_next, 1));
         if (rc || (rc = L.n_best_after(a, s2))) return rc;
+        if (timed && q % kGroupBlock == kGroupBlock - 1) HIPCHK(hipEventRecord(tev.back(), s2));
+      }
+      if (rec) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        GroupTune& t = g_group_tune[std::make_pair(L.g, njl)];
+        if (t.cv < 0 && t.ev.empty()) {
+          t.ev = std::move(tev);
+          t.trial_cv = std::move(tcv);
+        } else {
+          for (hipEvent_t e : tev) (void)hipEventDestroy(e);
+        }
       }
     }
     // jobs outside the shared launches: ER with a transmission constraint

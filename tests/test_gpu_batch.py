@@ -107,3 +107,25 @@ def test_batch_n_best_matches_alone(nd, synth, graph):
         for k in range(3):
             x, e = g.get_best_result(k)
             assert np.array_equal(x, want[k][0]) and e == want[k][1], k
+
+
+@pytest.mark.parametrize("double", [True, False], ids=["fp64", "fp32"])
+def test_shared_launch_variant_timing_keeps_results(nd, synth, double):
+    """A batch that is one launch group times the column tile variants on
+    the first 18 iterations of a call (blocks of three iterations per
+    variant) and uses the fastest from a later call on: every job's estimate
+    and errors stay bit-identical to the job run alone, through the timed
+    call, the call that settles the choice and the calls after it."""
+    counts = (20, 20, 7)
+    alone = []
+    for seed in (61, 62, 63):
+        est, g = planar_job(nd, synth, 512, seed, nd.HIO, double)
+        e = np.concatenate([g.iterate(c) for c in counts])
+        alone.append((est.get(), e))
+    jobs = [planar_job(nd, synth, 512, seed, nd.HIO, double) for seed in (61, 62, 63)]
+    b = nd.Batch([g for _, g in jobs])
+    e = np.vstack([b.iterate(c) for c in counts])
+    for j, (est, g) in enumerate(jobs):
+        x0, e0 = alone[j]
+        assert np.array_equal(est.get(), x0), j
+        assert np.array_equal(e[:, j], e0), j
