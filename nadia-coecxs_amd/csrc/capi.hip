@@ -103,7 +103,15 @@ struct DeviceCtx {
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
+  // page-locked host slots that the update pass writes each error into, one
+  // per reconstruction (get_error() after a stream sync reads its slot
+  // instead of copying from the device); allocated once, never freed,
+  // slots recycled
+  double* herr_pool = nullptr;  // host address
+  double* herr_dev = nullptr;   // the same memory as the device addresses it
+  std::vector<int> herr_free;
 };
+constexpr int kHostErrSlots = 8192;
 
 std::mutex g_mu;
 std::map<int, DeviceCtx> g_dev;
@@ -121,6 +129,40 @@ DeviceCtx* dev_ctx(int dev) {
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
   }
   return &c;
+}
+
+// A page-locked error slot of device context c: its host address (and
+// device address in *dev, index in *slot), or null when none is left or the
+// pool cannot be allocated (get_error() then copies from the device).
+double* herr_acquire(DeviceCtx* c, double** dev, int* slot) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!c->herr_pool) {
+    void* p = nullptr;
+    if (hipHostMalloc(&p, sizeof(double) * kHostErrSlots, hipHostMallocCoherent) != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    void* d = nullptr;
+    if (hipHostGetDevicePointer(&d, p, 0) != hipSuccess || !d) {
+      (void)hipGetLastError();
+      (void)hipHostFree(p);
+      return nullptr;
+    }
+    c->herr_pool = (double*)p;
+    c->herr_dev = (double*)d;
+    for (int i = kHostErrSlots - 1; i >= 0; --i) c->herr_free.push_back(i);
+  }
+  if (c->herr_free.empty()) return nullptr;
+  *slot = c->herr_free.back();
+  c->herr_free.pop_back();
+  *dev = c->herr_dev + *slot;
+  return c->herr_pool + *slot;
+}
+
+void herr_release(DeviceCtx* c, int slot) {
+  if (slot < 0) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  c->herr_free.push_back(slot);
 }
 
 // Host copy into the staging ring: large arrays are split over a few
@@ -539,6 +581,12 @@ struct nadia_recon {
   double* last_err;  // device slot holding get_error()
   double h_err_cache;
   bool h_err_valid;
+  // page-locked slot (device-visible) the update pass also writes each error
+  // into; herr_fresh: it holds get_error() once the stream's work is done
+  int herr_slot = -1;
+  double* herr = nullptr;      // host address
+  double* herr_dev = nullptr;  // device address
+  bool herr_fresh = false;
   // algorithm state (BaseCDI)
   double beta;
   int algorithm;
@@ -772,6 +820,7 @@ int op_scale_intensity(nadia_recon* r, void* buf, double* slot) {
   if (r->kind == NADIA_FRESNEL) DISPATCH(r->prec, k_fresnel_add<T>(buf, r->L, -1.0, r->npix, S_(r)));
   LAUNCHCHK();
   r->last_err = slot;
+  r->herr_fresh = false;
   r->h_err_valid = false;
   return 0;
 }
@@ -1028,6 +1077,7 @@ JobDesc make_jobdesc(const nadia_recon* r, void* X, void* W, double* errs) {
   // the fused update pass also writes each error into get_error()'s slot
   // (not the tuner's scratch descriptors, whose errors go to tune_errs)
   j.cur_err = errs == r->errs ? r->scal + 8 : nullptr;
+  j.host_err = errs == r->errs ? r->herr_dev : nullptr;
   for (int i = 0; i < 5; i++) j.coef[i] = r->alg[i];
   j.er = r->algorithm == NADIA_ER;
   j.store_every = wants_n_best(r) ? 1 : 0;
@@ -1285,6 +1335,7 @@ void fused_done(nadia_recon* r, int n) {
   // the ring
   const bool direct = fused_ok(r) && r->kind != NADIA_FRESNEL_WF;
   r->last_err = direct ? r->scal + 8 : r->errs + (n - 1);
+  r->herr_fresh = direct && r->herr;
   r->h_err_valid = false;
 }
 
@@ -1479,6 +1530,15 @@ int fetch_error(nadia_recon* r, double* out) {
     // current_error is never initialised by the reference before the first
     // projection; report 0
     *out = 0.0;
+    return 0;
+  }
+  if (r->herr_fresh) {
+    // the update pass wrote the error into the page-locked slot as well: no
+    // copy, only the wait
+    HIPCHK(hipStreamSynchronize(S_(r)));
+    r->h_err_cache = *(volatile const double*)r->herr;
+    r->h_err_valid = true;
+    *out = r->h_err_cache;
     return 0;
   }
   HIPCHK(hipMemcpyAsync(&r->h_err_cache, r->last_err, sizeof(double), hipMemcpyDeviceToHost, S_(r)));
@@ -1774,6 +1834,7 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   r->rsz = est->esz / 2;
   r->npix = (long long)r->nx * r->ny;
   r->ring_cap = 4096;
+  r->herr = herr_acquire(r->ctx, &r->herr_dev, &r->herr_slot);
   r->path = NADIA_PATH_FUSED;
   r->beta = 0.9;
   r->col_v = r->row_v = -1;
@@ -1890,6 +1951,7 @@ int nadia_recon_destroy(nadia_recon* r) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (void* p : r->best) (void)hipFree(p);
+  herr_release(r->ctx, r->herr_slot);
   delete r;
   return 0;
 }
@@ -2032,7 +2094,13 @@ int nadia_recon_iterate(nadia_recon* r, int n, double* errors) {
       r->est->version++;
     }
     if (errors) {
-      if (has_projection(r)) {
+      if (chunk == 1 && r->herr_fresh) {
+        // one iteration through the fused passes: its error is in the
+        // page-locked slot once the stream's work is done
+        HIPCHK(hipStreamSynchronize(S_(r)));
+        errors[done] = r->h_err_cache = *(volatile const double*)r->herr;
+        r->h_err_valid = true;
+      } else if (has_projection(r)) {
         HIPCHK(hipMemcpyAsync(errors + done, r->errs, chunk * sizeof(double), hipMemcpyDeviceToHost, S_(r)));
       } else {
         // a projection-free custom algorithm leaves get_error() unchanged
@@ -2048,6 +2116,7 @@ int nadia_recon_iterate(nadia_recon* r, int n, double* errors) {
   if (r->last_err && r->last_err != r->scal + 8) {
     HIPCHK(hipMemcpyAsync(r->scal + 8, r->last_err, sizeof(double), hipMemcpyDeviceToDevice, S_(r)));
     r->last_err = r->scal + 8;
+    r->herr_fresh = false;
   }
   return 0;
 }
@@ -4043,6 +4112,7 @@ int batch_run(nadia_batch* b, const std::vector<int>& counts, double* errors) {
       l.src[l.n] = r->last_err;
       l.dst[l.n] = r->scal + 8;
       r->last_err = r->scal + 8;
+      r->herr_fresh = false;
       if (++l.n == kCopyMax) {
         k_copy_doubles(l, c->stream);
         l.n = 0;

@@ -38,6 +38,7 @@ struct JobDesc {
   double* partials;   // this job's partial error sums
   double* errs;       // this job's error ring (slot = iteration index)
   double* cur_err;    // get_error()'s slot: the update pass writes each error there too (null: not)
+  double* host_err;   // a page-locked host slot it also writes (null: none), read after a stream sync
   double coef[5];     // PSF, PFS, PS, PF, PI
   int er;             // error-reduction special case
   int err_field;      // field whose projection error get_error() reports
