@@ -1834,7 +1834,6 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   r->rsz = est->esz / 2;
   r->npix = (long long)r->nx * r->ny;
   r->ring_cap = 4096;
-  r->herr = herr_acquire(r->ctx, &r->herr_dev, &r->herr_slot);
   r->path = NADIA_PATH_FUSED;
   r->beta = 0.9;
   r->col_v = r->row_v = -1;
@@ -1895,6 +1894,8 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   // BaseCDI::BaseCDI: algorithm = CUSTOM; set_algorithm(ER) (src/BaseCDI.c++:63-64)
   r->algorithm = NADIA_CUSTOM;
   nadia_recon_set_algorithm(r, NADIA_ER);
+  // last, so that no failure above leaves a slot taken
+  r->herr = herr_acquire(r->ctx, &r->herr_dev, &r->herr_slot);
   *out = r;
   return 0;
 }
