@@ -493,3 +493,41 @@ def test_consistent_state_is_a_fixed_point_2048(nd, synth, double, tol):
         assert rel(est.get().astype(np.complex128), x0) <= tol, alg
         assert e.max() <= tol * tol * 100, alg
         del g, est
+
+
+@pytest.mark.parametrize("shape", [(8, 8), (16, 16), (8, 32), (32, 8), (24, 48), (48, 24), (16, 96), (96, 16)])
+def test_small_native_shapes_follow_the_oracle(nd, oracle, shape):
+    """The shortest compiled lengths (tiles narrower than a wavefront, many
+    transforms per workgroup, rectangular fields): five FP64 HIO iterations
+    through the fused passes follow the oracle (estimate and every error
+    within 1e-12), and an FP32 run stays within 1e-5 of the FP64 oracle."""
+    nx, ny = shape
+    rng = np.random.default_rng(nx * 1000 + ny)
+    obj = np.zeros(shape, complex)
+    obj[nx // 4:3 * nx // 4, ny // 4:3 * ny // 4] = rng.uniform(0.5, 1, (3 * nx // 4 - nx // 4, 3 * ny // 4 - ny // 4))
+    I = np.abs(np.fft.fft2(obj)) ** 2 + 1e-3
+    sup = np.zeros(shape)
+    sup[nx // 8:7 * nx // 8, ny // 8:7 * ny // 8] = 1.0
+    x0 = sup * np.exp(2j * np.pi * rng.random(shape))
+    xo = x0.copy()
+    o = oracle.OracleCDI(xo, oracle.PLANAR)
+    o.set_support(sup)
+    o.set_intensity(I)
+    o.set_algorithm(oracle.HIO)
+    eo = o.iterate_n(5)
+    for double in (True, False):
+        cdt = np.complex128 if double else np.complex64
+        est = nd.Complex2D(nx, ny, double=double, data=x0.astype(cdt))
+        g = nd.PlanarCDI(est)
+        g.set_support(sup)
+        g.set_intensity(I)
+        g.set_algorithm(nd.HIO)
+        e = g.iterate(5)
+        x = est.get().astype(np.complex128)
+        if double:
+            assert rel(x, xo) <= 1e-12, shape
+            assert np.all(np.abs(e - eo) <= 1e-12 * np.abs(eo)), shape
+        else:
+            assert rel(x, xo) <= 1e-5, shape
+            assert np.all(np.abs(e - eo) <= 1e-5 * np.abs(eo)), shape
+        del g, est
