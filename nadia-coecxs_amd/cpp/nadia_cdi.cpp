@@ -318,10 +318,15 @@ void BaseCDI::convolve(Double_2D& array, double gauss_width, int cut) {
   array.copy(tmp);
 }
 
+// BaseCDI::apply_threshold (NADIA/src/BaseCDI.c++:477-492) loops j < nx: for
+// nx < ny the columns j >= nx keep their values; for nx > ny the reference's
+// loop writes past each row and, on the last row, past the array (undefined
+// behaviour), so here each pixel is then thresholded once, as on the device.
 void BaseCDI::apply_threshold(Double_2D& array, double threshold) {
   double max = array.get_max();
+  const int jn = nx < ny ? nx : ny;
   for (int i = 0; i < nx; i++)
-    for (int j = 0; j < nx; j++) array.set(i, j, array.get(i, j) < threshold * max ? 0.0 : 1.0);
+    for (int j = 0; j < jn; j++) array.set(i, j, array.get(i, j) < threshold * max ? 0.0 : 1.0);
 }
 
 // -------------------------------------------------------------- PlanarCDI --

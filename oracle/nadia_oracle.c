@@ -479,12 +479,16 @@ void otc_apply(otc *t, cplx *tr) {
 /* ---- FresnelCDI transmission helpers ------------------------------------ */
 
 /* FresnelCDI::set_transmission_function: NADIA/src/FresnelCDI.c++:253-283
- * (the loop runs j < nx: square arrays only, as in the reference) */
+ * (the loop runs j < nx, so for nx < ny the columns j >= nx keep their
+ * values; for nx > ny the reference's loop runs past each row and, on the
+ * last row, past the array -- undefined behaviour; here each pixel is then
+ * written once, as the device kernel does) */
 void ocdi_set_transmission_function(ocdi *o, const cplx *T, cplx *esw) {
   int nx = o->nx, ny = o->ny;
+  const int jn = nx < ny ? nx : ny;
   if (!esw) esw = o->complex;
   for (int i = 0; i < nx; i++)
-    for (int j = 0; j < nx; j++) {
+    for (int j = 0; j < jn; j++) {
       size_t p = (size_t)i * ny + j;
       double lr = o->illum_at_sample[p].re, li = o->illum_at_sample[p].im;
       double tr = T[p].re, ti = T[p].im;

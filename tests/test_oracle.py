@@ -771,3 +771,27 @@ def test_intensity_autocorrelation_is_object_autocorrelation(oracle):
     # centres it and divides by sqrt(N) = n
     expect = np.fft.fftshift(np.abs(R)) / n
     assert np.abs(got - expect).max() <= 1e-12 * expect.max()
+
+
+def test_set_transmission_function_stays_in_the_field(oracle):
+    """FresnelCDI::set_transmission_function loops j < nx
+    (NADIA/src/FresnelCDI.c++:270-283).  For nx < ny the columns j >= nx
+    keep their values; for nx > ny the reference's loop would run past the
+    array: the oracle writes every pixel once and nothing beyond the field
+    (guard elements after the estimate stay untouched)."""
+    rng = np.random.default_rng(23)
+    for nx, ny in ((32, 64), (64, 32)):
+        wf = rng.random((nx, ny)) + 1j * rng.random((nx, ny)) + 0.5
+        buf = np.full(nx * ny + 512, 7.0 + 7.0j)
+        x = buf[:nx * ny].reshape(nx, ny)
+        x[:] = 0
+        o = oracle.OracleCDI(x, oracle.FRESNEL)
+        o.fresnel_init(wf, 1.55e-10, 0.909, 1.6e-3, 13.5e-6, 1.0)
+        o.set_support(np.ones((nx, ny)))
+        o.set_intensity(np.abs(wf) ** 2)
+        o.initialise_estimate(0)
+        assert np.all(buf[nx * ny:] == 7.0 + 7.0j), (nx, ny)
+        if nx < ny:
+            assert np.all(x[:, nx:] == 0), (nx, ny)  # never written
+        else:
+            assert np.count_nonzero(x) > 0.9 * x.size
