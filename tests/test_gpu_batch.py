@@ -129,3 +129,39 @@ def test_shared_launch_variant_timing_keeps_results(nd, synth, double):
         x0, e0 = alone[j]
         assert np.array_equal(est.get(), x0), j
         assert np.array_equal(e[:, j], e0), j
+
+
+def test_rectangular_jobs_in_a_batch_match_alone(nd, synth):
+    """Jobs of transposed rectangular shapes (their own launch groups) and a
+    square one, three algorithms, FP64 and FP32: each bit-identical to the
+    same job run alone."""
+    def job(shape, seed, alg, double):
+        nx, ny = shape
+        rng = np.random.default_rng(seed)
+        obj = np.zeros(shape, complex)
+        obj[nx // 4:3 * nx // 4, ny // 4:3 * ny // 4] = rng.uniform(0.5, 1, (nx // 2, ny // 2))
+        I = np.abs(np.fft.fft2(obj)) ** 2 + 1e-3
+        sup = np.zeros(shape)
+        sup[nx // 8:7 * nx // 8, ny // 8:7 * ny // 8] = 1.0
+        cdt = np.complex128 if double else np.complex64
+        est = nd.Complex2D(nx, ny, double=double, data=(sup * np.exp(2j * np.pi * rng.random(shape))).astype(cdt))
+        g = nd.PlanarCDI(est)
+        g.set_support(sup)
+        g.set_intensity(I)
+        g.set_algorithm(alg)
+        return est, g
+
+    spec = [((256, 512), 71, nd.HIO, True), ((512, 256), 72, nd.RAAR, True), ((256, 512), 73, nd.ER, True),
+            ((256, 256), 74, nd.HIO, False), ((128, 768), 75, nd.HIO, False), ((768, 128), 76, nd.ER, False)]
+    alone = []
+    for shape, seed, alg, double in spec:
+        est, g = job(shape, seed, alg, double)
+        e = g.iterate(9)
+        alone.append((est.get(), e))
+    jobs = [job(*s) for s in spec]
+    b = nd.Batch([g for _, g in jobs])
+    e = np.vstack([b.iterate(4), b.iterate(5)])
+    for j, (est, g) in enumerate(jobs):
+        x0, e0 = alone[j]
+        assert np.array_equal(est.get(), x0), j
+        assert np.array_equal(e[:, j], e0), j
