@@ -345,6 +345,16 @@ def test_autocorrelation(nd, oracle, synth):
     d = small_state(synth, n=64)
     est, g, xo, o = make_pair(nd, oracle, d)
     assert rel(g.get_intensity_autocorrelation(), o.get_intensity_autocorrelation()) <= 1e-13
+    # rectangular patterns: the transform and the centring shift per axis
+    rng = np.random.default_rng(29)
+    for shape in ((64, 128), (128, 64)):
+        I = rng.uniform(0.0, 9.0, shape)
+        est = nd.Complex2D(*shape, double=True)
+        g = nd.PlanarCDI(est)
+        o = oracle.OracleCDI(np.zeros(shape, complex), oracle.PLANAR)
+        for obj in (g, o):
+            obj.set_intensity(I)
+        assert rel(g.get_intensity_autocorrelation(), o.get_intensity_autocorrelation()) <= 1e-13, shape
 
 
 def test_public_operators(nd, oracle, synth):
