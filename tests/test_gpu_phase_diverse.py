@@ -151,3 +151,52 @@ def test_fresnel_frames_with_constraint(nd, oracle, synth):
         assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-10, it
     for est, g, xo, o, tg, tcs in keep:
         assert rel(est.get(), xo) <= 1e-10
+
+
+@pytest.mark.parametrize("parallel", [False, True], ids=["series", "parallel"])
+@pytest.mark.parametrize("scale", [1, 2])
+def test_rectangular_planar_frames(nd, oracle, parallel, scale):
+    """Frames of 48 x 96 pixels at integral and fractional positions: the
+    object's bounding box, the weights and the frame <-> object mappings per
+    axis, against the oracle (errors 1e-9, object and frames 1e-10)."""
+    shape = (48, 96)
+    nx, ny = shape
+    pos = [(0, 0), (10.5, -13.25), (-7, 21)]
+    gpu, cpu = [], []
+    for k, _ in enumerate(pos):
+        rng = np.random.default_rng(40 + k)
+        obj = np.zeros(shape, complex)
+        obj[nx // 4:3 * nx // 4, ny // 4:3 * ny // 4] = rng.uniform(0.5, 1, (nx // 2, ny // 2))
+        I = np.abs(np.fft.fft2(obj)) ** 2 + 1e-3
+        sup = np.zeros(shape)
+        sup[nx // 8:7 * nx // 8, ny // 8:7 * ny // 8] = 1.0
+        x0 = sup * np.exp(2j * np.pi * rng.random(shape))
+        est = nd.Complex2D(nx, ny, double=True, data=x0)
+        g = nd.PlanarCDI(est)
+        xo = x0.copy()
+        o = oracle.OracleCDI(xo, oracle.PLANAR)
+        for obj_ in (g, o):
+            obj_.set_support(sup)
+            obj_.set_intensity(I)
+            obj_.set_algorithm(oracle.HIO)
+        gpu.append((est, g))
+        cpu.append((xo, o))
+    pg = nd.PhaseDiverseCDI(beta=0.8, gamma=1.0, parallel=parallel, granularity=scale)
+    po = oracle.OraclePhaseDiverse(beta=0.8, gamma=1.0, parallel=parallel, granularity=scale)
+    for (est, g), (xo, o), (x, y) in zip(gpu, cpu, pos):
+        pg.add_new_position(g, x, y, 1.0)
+        po.add_new_position(o, x, y, 1.0)
+    assert pg.size() == po.size()
+    pg.set_iterations_per_cycle(2)
+    po.set_iterations_per_cycle(2)
+    pg.initialise_estimate()
+    po.initialise_estimate()
+    assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-14
+    for it in range(3):
+        eg = pg.iterate()
+        po.iterate()
+        eo = np.array([o.get_error() for _, o in cpu])
+        assert (np.abs(eg - eo) / eo).max() <= 1e-9, it
+        assert rel(pg.get_transmission(), po.get_transmission()) <= 1e-10, it
+    for (est, g), (xo, o) in zip(gpu, cpu):
+        assert rel(est.get(), xo) <= 1e-10
