@@ -248,3 +248,43 @@ def test_planar_er_many_regions(nd, oracle, synth, nreg):
     assert rel(est.get(), xo) <= 1e-10
     for k in range(nreg):
         assert abs(tg.c_mean(k) - to.c_mean(k)) <= 1e-10 * abs(to.c_mean(k))
+
+
+@pytest.mark.parametrize("shape", [(128, 256), (256, 128)])
+@pytest.mark.parametrize("path", [0, 1], ids=["fused", "plain"])
+def test_rectangular_fresnel_with_constraint(nd, oracle, synth, shape, path):
+    """Constrained FresnelCDI ER on rectangular fields: the region map, the
+    transmission <-> exit-wave conversions (the reference's j < nx columns
+    for nx < ny, every pixel for nx > ny) and the fused constraint passes,
+    eight iterations within 1e-10 of the oracle."""
+    nx, ny = shape
+    d = synth.fresnel_config(n=256, pupil_diameter=160e-6 * 256 / 1024, disk_radius=20 * 256 / 1024 + 2,
+                             ndisks=4, support=500)
+    cut = lambda a: np.ascontiguousarray(a[:nx, :ny])
+    est = nd.Complex2D(nx, ny, double=True)
+    g = nd.FresnelCDI(est, cut(d["white_field"]).astype(np.complex128), d["wavelength"], d["zfd"], d["zfs"],
+                      d["pixel"], 1.0)
+    g.set_path(path)
+    xo = np.zeros(shape, np.complex128)
+    o = oracle.OracleCDI(xo, oracle.FRESNEL)
+    o.fresnel_init(cut(d["white_field"]), d["wavelength"], d["zfd"], d["zfs"], d["pixel"], 1.0)
+    for obj in (g, o):
+        obj.set_support(cut(d["support"]))
+        obj.set_intensity(cut(d["intensity"]))
+    tg = nd.TransmissionConstraint(nx, ny)
+    tcs = oracle.OracleConstraint(nx, ny)
+    inside = cut(d["support"])
+    for tc in (tg, tcs):
+        tc.add_complex_constraint(inside, 0.5, 0.5)
+    g.set_complex_constraint(tg)
+    o.set_complex_constraint(tcs)
+    rng = np.random.default_rng(6)
+    T0 = np.exp((0.3 + 1j) * -rng.uniform(0.1, 0.4, shape))
+    g.set_transmission_function(T0)
+    o.set_transmission_function(T0)
+    assert rel(est.get(), xo) <= 1e-13
+    eg = g.iterate(8)
+    eo = o.iterate_n(8)
+    assert (np.abs(eg - eo) / eo).max() <= 1e-10
+    assert rel(est.get(), xo) <= 1e-10
+    assert rel(g.get_transmission_function(), o.get_transmission_function()) <= 1e-10
