@@ -1851,7 +1851,7 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
     size_t total = 0;
     for (size_t b : len) total += (b + 255) & ~(size_t)255;
     if ((rc = alloc(&r->arena, total))) {
-      delete r;
+      nadia_recon_destroy(r);  // frees what was allocated so far
       return rc;
     }
     char* p = (char*)r->arena;
@@ -1868,7 +1868,7 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
   for (int i = 0; i < n_best; i++) {
     void* p = nullptr;
     if (hipMalloc(&p, cb) != hipSuccess) {
-      delete r;
+      nadia_recon_destroy(r);  // frees what was allocated so far
       return fail(NADIA_EHIP, "hipMalloc failed for n_best snapshot");
     }
     (void)hipMemsetAsync(p, 0, cb, S_(r));
@@ -1878,7 +1878,7 @@ int nadia_recon_create(nadia_field* est, int kind, int n_best, nadia_recon** out
     if ((rc = alloc((void**)&r->dbest_err, sizeof(double) * n_best)) ||
         (rc = alloc((void**)&r->dbest_perm, sizeof(int) * n_best)) ||
         (rc = alloc((void**)&r->dbest_bufs, sizeof(void*) * n_best)) || (rc = alloc((void**)&r->dbest_target, sizeof(int)))) {
-      delete r;
+      nadia_recon_destroy(r);  // frees what was allocated so far
       return rc;
     }
     // BaseCDI constructor: every best error starts at 1 (src/BaseCDI.c++:50-56)
